@@ -1,0 +1,162 @@
+"""Benchmark: fused D2Q9 step on the 8192x2048 von Karman channel, FP32 vs FP64.
+
+Workload (the precision configuration of the reference's channel scaled up):
+8192x2048 lattice, plate of 400 nodes at x = 2048 centred in y with a +1 node
+offset, inlet equilibrium u = (0.1, 0), rho = 1, outlet extrapolation, rigid
+top and bottom walls, nu = 0.1*400/100 (Re = 100) so omega = 1/1.7.
+
+value      : MLUPS of the FP32-deviation step (inputs resident in HBM).
+roofline   : HBM bound; achieved = algorithmic bytes per launch (18 population
+             accesses x 4 B per node) / average launch time over the timed region.
+fp64       : the same workload with FP64 populations.
+cpu_baseline: the oracle (numpy FP64, one core) on a few steps of the same workload.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS & %HBM roofline at 8192x2048 FP32 vs FP64; p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+LX, LY, PLATE = 8192, 2048, 400
+U_IN, RE = 0.1, 100.0
+BYTES_PER_NODE = {"f64": 144, "f32": 72, "f16": 36}
+
+
+def omega():
+    nu = U_IN * PLATE / RE
+    return 1.0 / (3.0 * nu + 0.5)
+
+
+def build_lattice(storage, torch):
+    from lattice_boltzmann_parallel_solver_amd import Lattice
+    from lattice_boltzmann_parallel_solver_amd.boundary_utils import von_karman_boundary_conditions
+
+    bc = von_karman_boundary_conditions(LX, LY, 1.0, U_IN, PLATE, walls=True, y_offset=1)
+    lat = Lattice(LX, LY, boundary=bc, storage=storage, stream=torch.cuda.current_stream().cuda_stream)
+    rho = torch.ones((LX, LY), dtype=torch.float64, device="cuda")
+    u = torch.zeros((LX, LY, 2), dtype=torch.float64, device="cuda")
+    u[..., 0] = U_IN
+    lat.set_equilibrium(rho, u)
+    del rho, u
+    return lat
+
+
+def time_steps(lat, steps, warmup, torch):
+    lat.run(omega(), warmup)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    lat.run(omega(), steps)
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end)  # ms
+
+
+def cpu_baseline(steps=2):
+    """numpy oracle (FP64, one core) on `steps` steps of the same workload after one warm-up step."""
+    import numpy as np
+
+    from oracle import lbm_oracle as O
+
+    rho, u = O.density_1_velocity_x_u0_velocity_y_0_initial((LX, LY), U_IN)
+    f = O.equilibrium(rho, u)
+    bc = O.von_karman_boundary_conditions(LX, LY, 1.0, U_IN, PLATE, walls=True, y_offset=1)
+    f, rho, u = O.step(f, rho, u, omega(), bc)
+    t0 = time.perf_counter_ns()
+    for _ in range(steps):
+        f, rho, u = O.step(f, rho, u, omega(), bc)
+    dt = (time.perf_counter_ns() - t0) / 1e9
+    assert np.isfinite(rho).all()
+    return {"value": LX * LY * steps / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps of the {LX}x{LY} FP64 channel with the numpy oracle "
+                      f"after 1 warm-up step ({dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "f64"])
+    ap.add_argument("--fp64-steps", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the lattice is not split across devices)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    N = LX * LY
+
+    lat = build_lattice(args.storage, torch)
+    ms = time_steps(lat, args.steps, args.warmup, torch)
+    m, jx, _ = lat.moments_sum()
+    assert m == m and abs(m / N - 1.0) < 0.1, "unstable run"
+    lat.close()
+    del lat
+    torch.cuda.empty_cache()
+    sec = ms / 1e3
+    mlups = N * args.steps / sec / 1e6
+    launch_s = sec / args.steps
+    bytes_launch = BYTES_PER_NODE[args.storage] * N
+    achieved = bytes_launch / launch_s / 1e9
+
+    fp64 = None
+    if args.fp64_steps > 0 and args.storage != "f64":
+        lat = build_lattice("f64", torch)
+        ms64 = time_steps(lat, args.fp64_steps, max(5, args.warmup // 4), torch)
+        lat.close()
+        del lat
+        torch.cuda.empty_cache()
+        s64 = ms64 / 1e3
+        ach64 = BYTES_PER_NODE["f64"] * N / (s64 / args.fp64_steps) / 1e9
+        fp64 = {"mlups": N * args.fp64_steps / s64 / 1e6, "ms_per_step": ms64 / args.fp64_steps,
+                "steps": args.fp64_steps, "roofline_achieved_gbs": ach64,
+                "roofline_frac": ach64 / HBM_PEAK_GBS, "bytes_per_node": BYTES_PER_NODE["f64"]}
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+
+    line = {
+        "metric": METRIC,
+        "value": mlups,
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f32": "f32", "f16": "f16-storage/f32", "f64": "f64"}[args.storage],
+        "data": "synthetic",
+        "config": {"workload": f"von Karman channel {LX}x{LY}, plate {PLATE} at x={LX // 4} (+1 y offset), "
+                               f"rigid walls, inlet u={U_IN}, Re={RE:g}, {args.storage} population storage",
+                   "lx": LX, "ly": LY, "storage": args.storage, "omega": omega(),
+                   "parallelism": "single GPU (whole lattice resident)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_node": BYTES_PER_NODE[args.storage]},
+        "fp64": fp64,
+        "latency": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

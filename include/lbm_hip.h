@@ -1,0 +1,168 @@
+/*
+ * lbm_hip.h - C ABI of the MI355X (gfx950) D2Q9 lattice-Boltzmann library.
+ *
+ * The reference solver (SimonSchrodi/lattice_boltzmann_parallel_solver) is
+ * numpy code with no native layer, so there is no C interface to mirror
+ * one-for-one.  Each entry point below names the reference Python function
+ * whose work it replaces (file:line in the reference tree); the Python
+ * package lattice_boltzmann_parallel_solver_amd keeps those functions'
+ * signatures and calls these entry points through ctypes.
+ *
+ * Conventions
+ *  - Every function returns int: LBM_OK (0) or a negative LBM_ERR_* code;
+ *    lbm_last_error() returns a thread-local message for the last failure.
+ *  - Arrays are C order with x slowest, as in the reference: populations
+ *    f[x][y][i] (nx*ny*9), density rho[x][y] (nx*ny), velocity u[x][y][k]
+ *    (nx*ny*2).  Inside a context the populations are stored as nine
+ *    y-contiguous planes instead; set/get_state convert.
+ *  - Pointers flagged LBM_HOST are host memory; LBM_DEVICE pointers are HIP
+ *    device memory on the context's device (e.g. a torch tensor's data_ptr).
+ *    Host pointers are borrowed for the duration of the call only.
+ *  - `stream` arguments are hipStream_t values (NULL = the null stream).
+ *  - A context is not re-entrant; distinct contexts are independent.
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define LBM_OK 0
+#define LBM_ERR_ARG -1      /* invalid argument: shape, omega, descriptor */
+#define LBM_ERR_HIP -2      /* a HIP runtime call failed */
+#define LBM_ERR_STATE -3    /* call not valid in the context's current state */
+#define LBM_ERR_NOMEM -4    /* host or device allocation failed */
+#define LBM_ERR_UNSTABLE -5 /* non-finite moments detected */
+
+/* ---- population storage inside a context ----------------------------- */
+#define LBM_STORE_F64 0 /* f_i in FP64, FP64 arithmetic in the reference's order */
+#define LBM_STORE_F32 1 /* deviations f_i - w_i in FP32, FP32 collide */
+#define LBM_STORE_F16 2 /* deviations f_i - w_i in FP16, FP32 collide */
+
+/* ---- element types of the stateless entry points --------------------- */
+#define LBM_DT_F64 0
+#define LBM_DT_F32 1
+
+/* ---- pointer location ------------------------------------------------- */
+#define LBM_HOST 0
+#define LBM_DEVICE 1
+
+/* ---- boundary descriptors --------------------------------------------- */
+#define LBM_BC_RIGID_WALL 1        /* boundary_conditions.py:78-113  rigid_wall(boundary) */
+#define LBM_BC_MOVING_WALL 2       /* boundary_conditions.py:170-214 moving_wall(boundary, u_w, avg_density) */
+#define LBM_BC_RIGID_OBJECT 3      /* boundary_conditions.py:116-167 rigid_object(boundary) */
+#define LBM_BC_INLET 4             /* boundary_conditions.py:217-254 inlet(shape, density_in, velocity_in) */
+#define LBM_BC_OUTLET 5            /* boundary_conditions.py:257-283 outlet() */
+#define LBM_BC_PRESSURE_PERIODIC 6 /* boundary_conditions.py:286-348 periodic_with_pressure_variations(boundary, p_in, p_out) */
+
+typedef struct lbm_bc_desc {
+    int32_t kind;         /* LBM_BC_* */
+    int32_t reserved;     /* must be 0 */
+    const uint8_t *mask;  /* host, nx*ny bytes, nonzero = node in the set; NULL for INLET/OUTLET */
+    double u_w[2];        /* MOVING_WALL: wall velocity */
+    double rho_w;         /* MOVING_WALL: wall density (the reference's avg_density) */
+    double rho_in;        /* INLET: density_in */
+    double u_in;          /* INLET: velocity_in (x component; y is 0) */
+    double p_in;          /* PRESSURE_PERIODIC: inlet pressure */
+    double p_out;         /* PRESSURE_PERIODIC: outlet pressure */
+} lbm_bc_desc;
+
+/* A boundary program: an ordered descriptor list compiled into per-node
+ * classes (last writer wins per node and direction, as in the reference's
+ * composites, boundary_utils.py:35-53, 87-108, 164-203).  Host-side object. */
+typedef struct lbm_bc_prog lbm_bc_prog;
+
+/* A simulation context: one lattice resident on one device. */
+typedef struct lbm_ctx lbm_ctx;
+
+/* ---- library ---------------------------------------------------------- */
+const char *lbm_last_error(void);
+int lbm_version(void);          /* returns the ABI version (>0) */
+int lbm_device_count(int *n);   /* number of visible HIP devices */
+
+/* ---- boundary programs (host only, no device needed) ------------------ */
+/* Compile descriptors for an nx*ny lattice.  Replaces the per-step work of the
+ * closures built by boundary_conditions.py:78-348 and boundary_utils.py:9-205. */
+int lbm_bc_compile(lbm_bc_prog **out, int nx, int ny, const lbm_bc_desc *descs, int n);
+int lbm_bc_destroy(lbm_bc_prog *prog);
+/* Sizes of the compiled tables: number of node classes (class 0 = plain
+ * fluid), number of nodes with a nonzero class, and summary flags (bit 0:
+ * outlet present, bit 1: pressure fix present). */
+int lbm_bc_query(const lbm_bc_prog *prog, int *nclasses, int64_t *nspecial, uint32_t *flags);
+/* Copy the compiled tables out (for inspection and host-side tests):
+ * codes[nx*ny] class per node; per class: src word (2 bits per direction:
+ * 0 stream, 1 reflect f*[opp] - k, 2 constant k, 3 outlet copy), flags word,
+ * k[9]; params = {rho_in, rho_out, D1 bits, D2 bits} of the pressure fix.
+ * Any output pointer may be NULL. */
+int lbm_bc_export(const lbm_bc_prog *prog, uint16_t *codes, uint32_t *class_src,
+                  uint32_t *class_flags, double *class_k, double *params);
+
+/* ---- contexts ----------------------------------------------------------- */
+/* Allocate an nx*ny lattice with the given storage on `device`. */
+int lbm_create(lbm_ctx **out, int nx, int ny, int storage, int device);
+int lbm_destroy(lbm_ctx *ctx);
+/* Enqueue the context's work on `hip_stream` (default: a stream the context
+ * creates at default priority). */
+int lbm_set_stream(lbm_ctx *ctx, void *hip_stream);
+/* Attach a compiled boundary program (copied to the device; prog may be
+ * destroyed afterwards).  NULL = fully periodic. */
+int lbm_set_bc(lbm_ctx *ctx, const lbm_bc_prog *prog);
+/* Load the step-input state (f, rho, u) of lattice_boltzmann_method.py:191.
+ * rho and u are used as given by the first collision, as the reference does. */
+int lbm_set_state(lbm_ctx *ctx, const double *f, const double *rho, const double *u, int where);
+/* Same with f = equilibrium(rho, u) (the reference's initialisation,
+ * e.g. experiments.py:686-687). */
+int lbm_set_state_equilibrium(lbm_ctx *ctx, const double *rho, const double *u, int where);
+/* Advance nsteps reference steps (lattice_boltzmann_method.py:191-228 with the
+ * attached boundary program).  Asynchronous on the context's stream. */
+int lbm_run(lbm_ctx *ctx, double omega, int64_t nsteps);
+/* Read the current state (f, rho, u) as the reference's step returns it.
+ * Synchronous.  Any output may be NULL. */
+int lbm_get_state(lbm_ctx *ctx, double *f, double *rho, double *u, int where);
+/* Record (rho, ux, uy) at node (px, py) after every step into a device ring
+ * of `capacity` samples (the probe of experiments.py:688-704).  capacity 0
+ * disables. Resets the ring. */
+int lbm_set_probe(lbm_ctx *ctx, int px, int py, int64_t capacity);
+/* Copy the probe samples recorded since lbm_set_probe, oldest first, at most
+ * `max` rows of (rho, ux, uy); *count receives the number copied. */
+int lbm_read_probe(lbm_ctx *ctx, double *out, int64_t max, int64_t *count);
+/* FP64 sums of rho, rho*ux, rho*uy over the current state (mass and
+ * momentum; tests/test_mass_preservation.py:33).  Synchronous. */
+int lbm_moments_sum(lbm_ctx *ctx, double out[3]);
+int lbm_synchronize(lbm_ctx *ctx);
+int lbm_steps_done(const lbm_ctx *ctx, int64_t *steps);
+/* Launch tuning of the fused step: cells per thread (1, 2 or 4), XCD-aware
+ * block order (0/1), non-temporal stores (0/1).  Defaults are tuned values. */
+int lbm_set_launch(lbm_ctx *ctx, int cells_per_thread, int xcd_remap, int nt_stores);
+
+/* ---- stateless drop-ins on device arrays (AoS, C order) ----------------- */
+/* streaming(), lattice_boltzmann_method.py:140-159: periodic shift, bit-exact. */
+int lbm_stream(const void *f_in, void *f_out, int nx, int ny, int dtype, void *stream);
+/* equilibrium_distr_func(), lattice_boltzmann_method.py:162-188. */
+int lbm_equilibrium(const void *rho, const void *u, void *feq, int64_t ncells, int dtype, void *stream);
+/* compute_density(), lattice_boltzmann_method.py:93-105. */
+int lbm_density(const void *f, void *rho, int64_t ncells, int dtype, void *stream);
+/* compute_velocity_field(), lattice_boltzmann_method.py:108-137 (uses the given rho). */
+int lbm_velocity(const void *rho, const void *f, void *u, int64_t ncells, int dtype, void *stream);
+/* BGK collision f + (equilibrium(rho, u) - f)*omega, lattice_boltzmann_method.py:213-215. */
+int lbm_collide(const void *f, const void *rho, const void *u, double omega, void *f_out,
+                int64_t ncells, int dtype, void *stream);
+/* Post-streaming boundary rules of a program applied in place to f_post
+ * (the closures of boundary_conditions.py:93-111, 138-165, 192-212, 239-252,
+ * 267-281).  f_prev is read only if the program has an outlet. FP64 only. */
+int lbm_bc_apply(const lbm_bc_prog *prog, const double *f_pre, double *f_post,
+                 const double *f_prev, void *stream);
+/* Pre-streaming pressure fix of a program applied in place to f_pre
+ * (boundary_conditions.py:320-346). FP64 only. */
+int lbm_bc_pressure(const lbm_bc_prog *prog, double *f_pre, const double *rho,
+                    const double *u, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H */

@@ -1,0 +1,620 @@
+// C ABI of liblbm_hip.so: contexts, state transfer, the step loop, probes,
+// reductions and the stateless drop-ins (declared in include/lbm_hip.h).
+//
+// A context keeps one lattice resident on one device.  Its phase follows the
+// reference's step (lattice_boltzmann_method.py:191-228):
+//   PENDING  the step input (f, rho, u) sits in the AoS FP64 staging buffers;
+//            the next step starts with a collision that uses rho, u as given.
+//   RUNNING  the planes hold the post-collision populations of step n-1
+//            (after the pre-streaming pressure fix); the reference's state f_n
+//            is stream+boundary of them and is materialised only on request.
+// A run of k steps is therefore 1 first-collision + (k-1) fused kernels
+// from PENDING, or k fused kernels from RUNNING.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "lbm_launch.h"
+
+namespace lbm {
+namespace {
+thread_local std::string g_err;
+}
+void set_error(const std::string &msg) { g_err = msg; }
+}  // namespace lbm
+
+using namespace lbm;
+
+#define LBM_HIP_CHECK(expr, what)                                                  \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            set_error(std::string(what) + ": " + hipGetErrorString(e_));           \
+            return LBM_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+
+namespace {
+
+enum Phase { kEmpty = 0, kPending = 1, kRunning = 2 };
+
+size_t plane_elem(int storage) { return storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2); }
+size_t side_elem(int storage) { return storage == LBM_STORE_F64 ? 8 : 4; }
+
+// One device allocation holding a boundary program's tables.
+struct BcUpload {
+    void *mem = nullptr;
+    BcView view;
+};
+
+hipError_t upload_bc(const lbm_bc_prog &p, BcUpload &up) {
+    const size_t N = p.codes.size();
+    const size_t o_codes = 0;
+    const size_t o_chunk = (o_codes + N * sizeof(uint16_t) + 255) & ~size_t(255);
+    const size_t o_cls = (o_chunk + p.chunk_flags.size() + 255) & ~size_t(255);
+    const size_t o_spec = (o_cls + p.classes.size() * sizeof(ClassEntry) + 255) & ~size_t(255);
+    const size_t total = o_spec + p.special.size() * sizeof(uint32_t) + 256;
+    char *mem = nullptr;
+    hipError_t e = hipMalloc(&mem, total);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(mem + o_codes, p.codes.data(), N * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(mem + o_chunk, p.chunk_flags.data(), p.chunk_flags.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(mem + o_cls, p.classes.data(), p.classes.size() * sizeof(ClassEntry), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !p.special.empty())
+        e = hipMemcpy(mem + o_spec, p.special.data(), p.special.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(mem);
+        return e;
+    }
+    up.mem = mem;
+    up.view.codes = reinterpret_cast<const uint16_t *>(mem + o_codes);
+    up.view.chunk = reinterpret_cast<const uint8_t *>(mem + o_chunk);
+    up.view.classes = reinterpret_cast<const ClassEntry *>(mem + o_cls);
+    up.view.special = reinterpret_cast<const uint32_t *>(mem + o_spec);
+    up.view.nspecial = (int64_t)p.special.size();
+    up.view.rho_in = p.rho_in;
+    up.view.rho_out = p.rho_out;
+    up.view.d1 = p.d1;
+    up.view.d2 = p.d2;
+    up.view.flags = p.flags;
+    return hipSuccess;
+}
+
+}  // namespace
+
+struct lbm_ctx {
+    int nx = 0, ny = 0, storage = 0, device = 0;
+    Geom g;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    StateView st;
+    double *stage_f = nullptr, *stage_rho = nullptr, *stage_u = nullptr;
+    BcUpload bc;
+    int cur = 0;
+    int phase = kEmpty;
+    int64_t steps = 0;
+    ProbeView probe;
+    int64_t probe_base = 0;
+    double *partials = nullptr;
+    double *sums = nullptr;
+    LaunchCfg cfg;
+};
+
+namespace {
+
+void release(lbm_ctx *c) {
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 2; k++) {
+        (void)hipFree(c->st.plane[k]);
+        (void)hipFree(c->st.side[k]);
+    }
+    (void)hipFree(c->stage_f);
+    (void)hipFree(c->stage_rho);
+    (void)hipFree(c->stage_u);
+    (void)hipFree(c->bc.mem);
+    (void)hipFree(c->probe.ring);
+    (void)hipFree(c->partials);
+    (void)hipFree(c->sums);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+int check_ctx(lbm_ctx *c, const char *fn) {
+    if (!c) {
+        set_error(std::string(fn) + ": context is NULL");
+        return LBM_ERR_ARG;
+    }
+    LBM_HIP_CHECK(hipSetDevice(c->device), fn);
+    return LBM_OK;
+}
+
+hipMemcpyKind kind_in(int where) { return where == LBM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice; }
+hipMemcpyKind kind_out(int where) { return where == LBM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost; }
+
+}  // namespace
+
+extern "C" {
+
+const char *lbm_last_error(void) { return g_err.c_str(); }
+
+int lbm_version(void) { return 1; }
+
+int lbm_device_count(int *n) {
+    if (!n) {
+        set_error("lbm_device_count: n is NULL");
+        return LBM_ERR_ARG;
+    }
+    *n = 0;
+    LBM_HIP_CHECK(hipGetDeviceCount(n), "hipGetDeviceCount");
+    return LBM_OK;
+}
+
+int lbm_bc_destroy(lbm_bc_prog *prog) {
+    if (prog && prog->dev_mem) {
+        (void)hipSetDevice(prog->dev_id);
+        (void)hipFree(prog->dev_mem);
+    }
+    delete prog;
+    return LBM_OK;
+}
+
+int lbm_create(lbm_ctx **out, int nx, int ny, int storage, int device) {
+    if (!out) {
+        set_error("lbm_create: out is NULL");
+        return LBM_ERR_ARG;
+    }
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || (int64_t)nx * ny > 0x7fffffffLL) {
+        set_error("lbm_create: lattice must satisfy 1 <= nx*ny < 2^31");
+        return LBM_ERR_ARG;
+    }
+    if (storage < LBM_STORE_F64 || storage > LBM_STORE_F16) {
+        set_error("lbm_create: unknown storage " + std::to_string(storage));
+        return LBM_ERR_ARG;
+    }
+    int ndev = 0;
+    LBM_HIP_CHECK(hipGetDeviceCount(&ndev), "lbm_create: hipGetDeviceCount");
+    if (device < 0 || device >= ndev) {
+        set_error("lbm_create: device " + std::to_string(device) + " not present (" + std::to_string(ndev) +
+                  " visible)");
+        return LBM_ERR_ARG;
+    }
+    auto *c = new (std::nothrow) lbm_ctx();
+    if (!c) {
+        set_error("lbm_create: out of host memory");
+        return LBM_ERR_NOMEM;
+    }
+    c->nx = nx;
+    c->ny = ny;
+    c->storage = storage;
+    c->device = device;
+    c->g = make_geom((uint32_t)nx, (uint32_t)ny);
+    const size_t N = (size_t)nx * ny;
+    hipError_t e = hipSetDevice(device);
+    auto alloc = [&](void **p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, bytes);
+    };
+    for (int k = 0; k < 2; k++) {
+        alloc(&c->st.plane[k], 9 * N * plane_elem(storage));
+        alloc(&c->st.side[k], 3 * (size_t)ny * side_elem(storage));
+    }
+    alloc(reinterpret_cast<void **>(&c->stage_f), 9 * N * sizeof(double));
+    alloc(reinterpret_cast<void **>(&c->stage_rho), N * sizeof(double));
+    alloc(reinterpret_cast<void **>(&c->stage_u), 2 * N * sizeof(double));
+    alloc(reinterpret_cast<void **>(&c->partials), 3 * (size_t)partial_rows_needed(c->g, LaunchCfg{1, 1, 0}) *
+                                                       sizeof(double));
+    alloc(reinterpret_cast<void **>(&c->sums), 3 * sizeof(double));
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(c->st.side[k], 0, 3 * (size_t)ny * side_elem(storage));
+    if (e == hipSuccess) {
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        c->own_stream = e == hipSuccess;
+    }
+    if (e != hipSuccess) {
+        set_error(std::string("lbm_create: ") + hipGetErrorString(e));
+        release(c);
+        delete c;
+        return e == hipErrorOutOfMemory ? LBM_ERR_NOMEM : LBM_ERR_HIP;
+    }
+    *out = c;
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_ctx *c) {
+    if (!c) return LBM_OK;
+    release(c);
+    delete c;
+    return LBM_OK;
+}
+
+int lbm_set_stream(lbm_ctx *c, void *s) {
+    int rc = check_ctx(c, "lbm_set_stream");
+    if (rc) return rc;
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_set_stream");
+    if (c->own_stream) LBM_HIP_CHECK(hipStreamDestroy(c->stream), "lbm_set_stream");
+    c->stream = static_cast<hipStream_t>(s);
+    c->own_stream = false;
+    return LBM_OK;
+}
+
+int lbm_set_launch(lbm_ctx *c, int v, int xcd, int nt) {
+    int rc = check_ctx(c, "lbm_set_launch");
+    if (rc) return rc;
+    if (v != 1 && v != 2 && v != 4) {
+        set_error("lbm_set_launch: cells_per_thread must be 1, 2 or 4");
+        return LBM_ERR_ARG;
+    }
+    c->cfg.cells_per_thread = v;
+    c->cfg.xcd_remap = xcd ? 1 : 0;
+    c->cfg.nt_stores = nt ? 1 : 0;
+    return LBM_OK;
+}
+
+// Materialise f_n, rho_n, u_n of a running context into (device) outputs.
+static int finalize_into(lbm_ctx *c, double *f, double *rho, double *u) {
+    LBM_HIP_CHECK(launch_finalize(c->storage, c->g, c->st, c->cur, c->bc.view, f, rho, u, nullptr, c->stream,
+                                  nullptr),
+                  "finalize kernel");
+    return LBM_OK;
+}
+
+int lbm_set_bc(lbm_ctx *c, const lbm_bc_prog *prog) {
+    int rc = check_ctx(c, "lbm_set_bc");
+    if (rc) return rc;
+    if (prog && (prog->nx != c->nx || prog->ny != c->ny)) {
+        set_error("lbm_set_bc: program compiled for " + std::to_string(prog->nx) + "x" + std::to_string(prog->ny) +
+                  ", context is " + std::to_string(c->nx) + "x" + std::to_string(c->ny));
+        return LBM_ERR_ARG;
+    }
+    if (c->phase == kRunning) {
+        // the boundary changes what the lazily-held f_n is; fix f_n first
+        rc = finalize_into(c, c->stage_f, c->stage_rho, c->stage_u);
+        if (rc) return rc;
+        c->phase = kPending;
+    }
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_set_bc");
+    (void)hipFree(c->bc.mem);
+    c->bc = BcUpload{};
+    if (prog) LBM_HIP_CHECK(upload_bc(*prog, c->bc), "lbm_set_bc: upload");
+    return LBM_OK;
+}
+
+int lbm_set_state(lbm_ctx *c, const double *f, const double *rho, const double *u, int where) {
+    int rc = check_ctx(c, "lbm_set_state");
+    if (rc) return rc;
+    if (!f || !rho || !u) {
+        set_error("lbm_set_state: f, rho and u are required");
+        return LBM_ERR_ARG;
+    }
+    const size_t N = c->g.N;
+    LBM_HIP_CHECK(hipMemcpyAsync(c->stage_f, f, 9 * N * sizeof(double), kind_in(where), c->stream), "lbm_set_state");
+    LBM_HIP_CHECK(hipMemcpyAsync(c->stage_rho, rho, N * sizeof(double), kind_in(where), c->stream), "lbm_set_state");
+    LBM_HIP_CHECK(hipMemcpyAsync(c->stage_u, u, 2 * N * sizeof(double), kind_in(where), c->stream), "lbm_set_state");
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_set_state");
+    c->phase = kPending;
+    c->steps = 0;
+    c->probe_base = 0;
+    return LBM_OK;
+}
+
+int lbm_set_state_equilibrium(lbm_ctx *c, const double *rho, const double *u, int where) {
+    int rc = check_ctx(c, "lbm_set_state_equilibrium");
+    if (rc) return rc;
+    if (!rho || !u) {
+        set_error("lbm_set_state_equilibrium: rho and u are required");
+        return LBM_ERR_ARG;
+    }
+    const size_t N = c->g.N;
+    LBM_HIP_CHECK(hipMemcpyAsync(c->stage_rho, rho, N * sizeof(double), kind_in(where), c->stream),
+                  "lbm_set_state_equilibrium");
+    LBM_HIP_CHECK(hipMemcpyAsync(c->stage_u, u, 2 * N * sizeof(double), kind_in(where), c->stream),
+                  "lbm_set_state_equilibrium");
+    LBM_HIP_CHECK(launch_equilibrium_aos(c->stage_rho, c->stage_u, c->stage_f, (int64_t)N, LBM_DT_F64, c->stream),
+                  "equilibrium kernel");
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_set_state_equilibrium");
+    c->phase = kPending;
+    c->steps = 0;
+    c->probe_base = 0;
+    return LBM_OK;
+}
+
+int lbm_run(lbm_ctx *c, double omega, int64_t nsteps) {
+    int rc = check_ctx(c, "lbm_run");
+    if (rc) return rc;
+    if (!(omega > 0.0 && omega < 2.0)) {  // lattice_boltzmann_method.py:212
+        set_error("lbm_run: omega must satisfy 0 < omega < 2");
+        return LBM_ERR_ARG;
+    }
+    if (nsteps < 0) {
+        set_error("lbm_run: nsteps must be >= 0");
+        return LBM_ERR_ARG;
+    }
+    if (c->phase == kEmpty) {
+        set_error("lbm_run: no state loaded (call lbm_set_state first)");
+        return LBM_ERR_STATE;
+    }
+    if (nsteps == 0) return LBM_OK;
+    int64_t left = nsteps;
+    if (c->phase == kPending) {
+        LBM_HIP_CHECK(launch_collide_first(c->storage, c->g, c->stage_f, c->stage_rho, c->stage_u, c->st, c->cur,
+                                           c->bc.view, omega, c->stream),
+                      "first-collision kernel");
+        c->phase = kRunning;
+        c->steps++;
+        left--;
+    }
+    const bool probing = c->probe.cell >= 0;
+    for (; left > 0; left--) {
+        // this launch materialises f_{steps} in registers: record it if probed
+        const int64_t slot = probing ? c->steps - c->probe_base - 1 : -1;
+        LBM_HIP_CHECK(launch_fused(c->storage, c->g, c->st, c->cur, c->bc.view, omega, c->probe, slot, nullptr,
+                                   c->cfg, c->stream, nullptr),
+                      "fused step kernel");
+        c->cur ^= 1;
+        c->steps++;
+    }
+    if (probing && c->steps > c->probe_base)
+        LBM_HIP_CHECK(launch_probe_cell(c->storage, c->g, c->st, c->cur, c->bc.view, c->probe,
+                                        c->steps - c->probe_base - 1, c->stream),
+                      "probe kernel");
+    return LBM_OK;
+}
+
+int lbm_get_state(lbm_ctx *c, double *f, double *rho, double *u, int where) {
+    int rc = check_ctx(c, "lbm_get_state");
+    if (rc) return rc;
+    if (c->phase == kEmpty) {
+        set_error("lbm_get_state: no state loaded");
+        return LBM_ERR_STATE;
+    }
+    const size_t N = c->g.N;
+    if (c->phase == kRunning) {
+        if (where == LBM_DEVICE) {
+            rc = finalize_into(c, f, rho, u);
+            if (rc) return rc;
+            LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_get_state");
+            return LBM_OK;
+        }
+        rc = finalize_into(c, f ? c->stage_f : nullptr, rho ? c->stage_rho : nullptr, u ? c->stage_u : nullptr);
+        if (rc) return rc;
+    }
+    // staging now holds f_n (PENDING: the loaded input)
+    if (f) LBM_HIP_CHECK(hipMemcpyAsync(f, c->stage_f, 9 * N * sizeof(double), kind_out(where), c->stream), "lbm_get_state");
+    if (rho) LBM_HIP_CHECK(hipMemcpyAsync(rho, c->stage_rho, N * sizeof(double), kind_out(where), c->stream), "lbm_get_state");
+    if (u) LBM_HIP_CHECK(hipMemcpyAsync(u, c->stage_u, 2 * N * sizeof(double), kind_out(where), c->stream), "lbm_get_state");
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_get_state");
+    return LBM_OK;
+}
+
+int lbm_set_probe(lbm_ctx *c, int px, int py, int64_t capacity) {
+    int rc = check_ctx(c, "lbm_set_probe");
+    if (rc) return rc;
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_set_probe");
+    (void)hipFree(c->probe.ring);
+    c->probe = ProbeView{};
+    if (capacity <= 0) return LBM_OK;
+    if (px < 0 || px >= c->nx || py < 0 || py >= c->ny) {
+        set_error("lbm_set_probe: probe node outside the lattice");
+        return LBM_ERR_ARG;
+    }
+    LBM_HIP_CHECK(hipMalloc(&c->probe.ring, 3 * (size_t)capacity * sizeof(double)), "lbm_set_probe");
+    c->probe.capacity = capacity;
+    c->probe.cell = (int64_t)px * c->ny + py;
+    c->probe_base = c->steps;
+    return LBM_OK;
+}
+
+int lbm_read_probe(lbm_ctx *c, double *out, int64_t max, int64_t *count) {
+    int rc = check_ctx(c, "lbm_read_probe");
+    if (rc) return rc;
+    if (count) *count = 0;
+    if (c->probe.cell < 0) {
+        set_error("lbm_read_probe: no probe set");
+        return LBM_ERR_STATE;
+    }
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_read_probe");
+    const int64_t recorded = c->steps - c->probe_base;
+    const int64_t avail = recorded < c->probe.capacity ? recorded : c->probe.capacity;
+    const int64_t n = avail < max ? avail : max;
+    if (n <= 0 || !out) return LBM_OK;
+    // oldest kept sample first; copy the tail of the window
+    const int64_t first = recorded - n;
+    for (int64_t k = 0; k < n;) {
+        const int64_t slot = (first + k) % c->probe.capacity;
+        const int64_t run = std::min<int64_t>(n - k, c->probe.capacity - slot);
+        LBM_HIP_CHECK(hipMemcpy(out + 3 * k, c->probe.ring + 3 * slot, 3 * (size_t)run * sizeof(double),
+                                hipMemcpyDeviceToHost),
+                      "lbm_read_probe");
+        k += run;
+    }
+    if (count) *count = n;
+    return LBM_OK;
+}
+
+int lbm_moments_sum(lbm_ctx *c, double out[3]) {
+    int rc = check_ctx(c, "lbm_moments_sum");
+    if (rc) return rc;
+    if (!out) {
+        set_error("lbm_moments_sum: out is NULL");
+        return LBM_ERR_ARG;
+    }
+    int nb = 0;
+    if (c->phase == kRunning) {
+        LBM_HIP_CHECK(launch_finalize(c->storage, c->g, c->st, c->cur, c->bc.view, nullptr, nullptr, nullptr,
+                                      c->partials, c->stream, &nb),
+                      "moment-sum kernel");
+    } else if (c->phase == kPending) {
+        LBM_HIP_CHECK(launch_moments_sum_aos(c->stage_f, (int64_t)c->g.N, c->partials, c->stream, &nb),
+                      "moment-sum kernel");
+    } else {
+        set_error("lbm_moments_sum: no state loaded");
+        return LBM_ERR_STATE;
+    }
+    LBM_HIP_CHECK(launch_sum_partials(c->partials, nb, c->sums, c->stream), "partial-sum kernel");
+    LBM_HIP_CHECK(hipMemcpyAsync(out, c->sums, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                  "lbm_moments_sum");
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_moments_sum");
+    if (!(std::isfinite(out[0]) && std::isfinite(out[1]) && std::isfinite(out[2]))) {
+        set_error("lbm_moments_sum: non-finite moments (the run is unstable)");
+        return LBM_ERR_UNSTABLE;
+    }
+    return LBM_OK;
+}
+
+int lbm_synchronize(lbm_ctx *c) {
+    int rc = check_ctx(c, "lbm_synchronize");
+    if (rc) return rc;
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_synchronize");
+    return LBM_OK;
+}
+
+int lbm_steps_done(const lbm_ctx *c, int64_t *steps) {
+    if (!c || !steps) {
+        set_error("lbm_steps_done: NULL argument");
+        return LBM_ERR_ARG;
+    }
+    *steps = c->steps;
+    return LBM_OK;
+}
+
+// ---- stateless drop-ins ---------------------------------------------------
+
+static int check_dtype(int dtype, const char *fn) {
+    if (dtype != LBM_DT_F64 && dtype != LBM_DT_F32) {
+        set_error(std::string(fn) + ": dtype must be LBM_DT_F64 or LBM_DT_F32");
+        return LBM_ERR_ARG;
+    }
+    return LBM_OK;
+}
+
+int lbm_stream(const void *f_in, void *f_out, int nx, int ny, int dtype, void *stream) {
+    if (int rc = check_dtype(dtype, "lbm_stream")) return rc;
+    if (nx < 1 || ny < 1 || !f_in || !f_out || f_in == f_out) {
+        set_error("lbm_stream: need nx, ny >= 1 and distinct non-NULL buffers");
+        return LBM_ERR_ARG;
+    }
+    LBM_HIP_CHECK(launch_stream_aos(f_in, f_out, nx, ny, dtype, static_cast<hipStream_t>(stream)), "lbm_stream");
+    return LBM_OK;
+}
+
+int lbm_equilibrium(const void *rho, const void *u, void *feq, int64_t n, int dtype, void *stream) {
+    if (int rc = check_dtype(dtype, "lbm_equilibrium")) return rc;
+    if (n < 0 || (n > 0 && (!rho || !u || !feq))) {
+        set_error("lbm_equilibrium: bad arguments");
+        return LBM_ERR_ARG;
+    }
+    if (n == 0) return LBM_OK;
+    LBM_HIP_CHECK(launch_equilibrium_aos(rho, u, feq, n, dtype, static_cast<hipStream_t>(stream)), "lbm_equilibrium");
+    return LBM_OK;
+}
+
+int lbm_density(const void *f, void *rho, int64_t n, int dtype, void *stream) {
+    if (int rc = check_dtype(dtype, "lbm_density")) return rc;
+    if (n < 0 || (n > 0 && (!f || !rho))) {
+        set_error("lbm_density: bad arguments");
+        return LBM_ERR_ARG;
+    }
+    if (n == 0) return LBM_OK;
+    LBM_HIP_CHECK(launch_density_aos(f, rho, n, dtype, static_cast<hipStream_t>(stream)), "lbm_density");
+    return LBM_OK;
+}
+
+int lbm_velocity(const void *rho, const void *f, void *u, int64_t n, int dtype, void *stream) {
+    if (int rc = check_dtype(dtype, "lbm_velocity")) return rc;
+    if (n < 0 || (n > 0 && (!f || !rho || !u))) {
+        set_error("lbm_velocity: bad arguments");
+        return LBM_ERR_ARG;
+    }
+    if (n == 0) return LBM_OK;
+    LBM_HIP_CHECK(launch_velocity_aos(rho, f, u, n, dtype, static_cast<hipStream_t>(stream)), "lbm_velocity");
+    return LBM_OK;
+}
+
+int lbm_collide(const void *f, const void *rho, const void *u, double omega, void *f_out, int64_t n, int dtype,
+                void *stream) {
+    if (int rc = check_dtype(dtype, "lbm_collide")) return rc;
+    if (!(omega > 0.0 && omega < 2.0)) {
+        set_error("lbm_collide: omega must satisfy 0 < omega < 2");
+        return LBM_ERR_ARG;
+    }
+    if (n < 0 || (n > 0 && (!f || !rho || !u || !f_out))) {
+        set_error("lbm_collide: bad arguments");
+        return LBM_ERR_ARG;
+    }
+    if (n == 0) return LBM_OK;
+    LBM_HIP_CHECK(launch_collide_aos(f, rho, u, omega, f_out, n, dtype, static_cast<hipStream_t>(stream)),
+                  "lbm_collide");
+    return LBM_OK;
+}
+
+static int prog_on_device(const lbm_bc_prog *prog, BcView &view, const char *fn) {
+    int dev = 0;
+    LBM_HIP_CHECK(hipGetDevice(&dev), fn);
+    auto *p = const_cast<lbm_bc_prog *>(prog);
+    if (p->dev_mem && p->dev_id != dev) {
+        (void)hipSetDevice(p->dev_id);
+        (void)hipFree(p->dev_mem);
+        (void)hipSetDevice(dev);
+        p->dev_mem = nullptr;
+    }
+    if (!p->dev_mem) {
+        BcUpload up;
+        LBM_HIP_CHECK(upload_bc(*p, up), fn);
+        p->dev_mem = up.mem;
+        p->dev_id = dev;
+        p->dev_codes = up.view.codes;
+        p->dev_chunk = up.view.chunk;
+        p->dev_classes = up.view.classes;
+        p->dev_special = up.view.special;
+    }
+    view.codes = p->dev_codes;
+    view.chunk = p->dev_chunk;
+    view.classes = p->dev_classes;
+    view.special = p->dev_special;
+    view.nspecial = (int64_t)p->special.size();
+    view.rho_in = p->rho_in;
+    view.rho_out = p->rho_out;
+    view.d1 = p->d1;
+    view.d2 = p->d2;
+    view.flags = p->flags;
+    return LBM_OK;
+}
+
+int lbm_bc_apply(const lbm_bc_prog *prog, const double *f_pre, double *f_post, const double *f_prev, void *stream) {
+    if (!prog || !f_pre || !f_post) {
+        set_error("lbm_bc_apply: prog, f_pre and f_post are required");
+        return LBM_ERR_ARG;
+    }
+    if ((prog->flags & kProgOutlet) && !f_prev) {
+        set_error("lbm_bc_apply: the program has an outlet, f_prev is required");
+        return LBM_ERR_ARG;
+    }
+    BcView view;
+    if (int rc = prog_on_device(prog, view, "lbm_bc_apply")) return rc;
+    const Geom g = make_geom((uint32_t)prog->nx, (uint32_t)prog->ny);
+    LBM_HIP_CHECK(launch_bc_apply(g, view, f_pre, f_post, f_prev, static_cast<hipStream_t>(stream)), "lbm_bc_apply");
+    return LBM_OK;
+}
+
+int lbm_bc_pressure(const lbm_bc_prog *prog, double *f_pre, const double *rho, const double *u, void *stream) {
+    if (!prog || !f_pre || !rho || !u) {
+        set_error("lbm_bc_pressure: prog, f_pre, rho and u are required");
+        return LBM_ERR_ARG;
+    }
+    if (!(prog->flags & kProgPressure)) {
+        set_error("lbm_bc_pressure: the program has no pressure-periodic descriptor");
+        return LBM_ERR_ARG;
+    }
+    BcView view;
+    if (int rc = prog_on_device(prog, view, "lbm_bc_pressure")) return rc;
+    const Geom g = make_geom((uint32_t)prog->nx, (uint32_t)prog->ny);
+    LBM_HIP_CHECK(launch_bc_pressure(g, view, f_pre, rho, u, static_cast<hipStream_t>(stream)), "lbm_bc_pressure");
+    return LBM_OK;
+}
+
+}  // extern "C"

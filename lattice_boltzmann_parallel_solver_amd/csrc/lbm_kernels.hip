@@ -1,0 +1,842 @@
+// D2Q9 lattice-Boltzmann kernels for gfx950 (MI355X).
+//
+// State layout in HBM: nine y-contiguous population planes per buffer,
+// plane i at base + i*N, node n = x*ly + y (x slowest, as in the reference's
+// f[x, y, i]); two buffers (A/B) alternate every step.  The hot kernel is a
+// pull-scheme fused step: each thread gathers the nine populations streaming
+// into its node, applies the node's boundary class, computes the moments,
+// the equilibrium and the BGK relaxation in registers, and writes the nine
+// post-collision populations.  It is bandwidth bound (18 plane accesses per
+// node per step, no reuse inside a step), so the design goal is one
+// coalesced, aligned store and one coalesced load per plane per node.
+//
+// Storage policies
+//   double : f_i in FP64, arithmetic in the reference's exact operation order
+//            (lattice_boltzmann_method.py:93-215) with FMA contraction off.
+//   float  : deviations d_i = f_i - w_i in FP32, FP32 arithmetic.
+//   __half : deviations in FP16, FP32 arithmetic.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "lbm_launch.h"
+
+namespace lbm {
+
+Geom make_geom(uint32_t lx, uint32_t ly) {
+    Geom g;
+    g.lx = lx;
+    g.ly = ly;
+    g.N = lx * ly;
+    g.ly_magic = ly == 1 ? 0 : (~0ull) / ly + 1;
+    return g;
+}
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kWd[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9,
+                           1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+
+// ---------------------------------------------------------------------------
+// storage I/O
+// ---------------------------------------------------------------------------
+template <class S>
+struct Io;
+
+template <>
+struct Io<double> {
+    using C = double;
+    static constexpr bool kDev = false;
+    __device__ static C ld(const double *p) { return *p; }
+    __device__ static void st(double *p, C v, bool nt) {
+        if (nt) __builtin_nontemporal_store(v, p);
+        else *p = v;
+    }
+};
+
+template <>
+struct Io<float> {
+    using C = float;
+    static constexpr bool kDev = true;
+    __device__ static C ld(const float *p) { return *p; }
+    __device__ static void st(float *p, C v, bool nt) {
+        if (nt) __builtin_nontemporal_store(v, p);
+        else *p = v;
+    }
+};
+
+template <>
+struct Io<__half> {
+    using C = float;
+    static constexpr bool kDev = true;
+    __device__ static C ld(const __half *p) { return __half2float(*p); }
+    __device__ static void st(__half *p, C v, bool nt) {
+        unsigned short bits = __half_as_ushort(__float2half_rn(v));
+        if (nt) __builtin_nontemporal_store(bits, reinterpret_cast<unsigned short *>(p));
+        else *reinterpret_cast<unsigned short *>(p) = bits;
+    }
+};
+
+// Representation of a float64 population value f_i in the compute type of S
+// (deviation f - w_i for the reduced-precision policies).
+template <class S>
+__device__ __forceinline__ typename Io<S>::C rep(double f, int i) {
+    if constexpr (Io<S>::kDev) return (typename Io<S>::C)(f - kWd[i]);
+    else return f;
+}
+
+// ---------------------------------------------------------------------------
+// node index helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t row_of(const Geom &g, uint32_t n) {
+    return g.ly == 1 ? n : (uint32_t)__umul64hi(g.ly_magic, (uint64_t)n);
+}
+
+// Spread consecutive block ids over the 8 XCDs so that each XCD works on one
+// contiguous range of nodes; the y-shifted loads that straddle two blocks
+// then hit the same L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
+    const uint32_t q = G >> 3, r = G & 7, xcd = b & 7, local = b >> 3;
+    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+// Periodic pull: population i of node (x, y) comes from node (x - cx_i, y - cy_i).
+template <class S>
+__device__ __forceinline__ void pull(const S *__restrict__ src, const Geom &g, uint32_t n, uint32_t x,
+                                     uint32_t y, typename Io<S>::C out[9]) {
+    const bool x0 = x == 0, x1 = x == g.lx - 1, y0 = y == 0, y1 = y == g.ly - 1;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        uint32_t m = n - (uint32_t)(kCX[i] * (int)g.ly + kCY[i]);
+        if (kCX[i] == 1 && x0) m += g.N;
+        if (kCX[i] == -1 && x1) m -= g.N;
+        if (kCY[i] == 1 && y0) m += g.ly;
+        if (kCY[i] == -1 && y1) m -= g.ly;
+        out[i] = Io<S>::ld(src + (size_t)i * g.N + m);
+    }
+}
+
+// Apply the node's boundary class to the pulled populations; returns the
+// class flags (0 for plain fluid).
+template <class S>
+__device__ __forceinline__ uint32_t apply_class(const BcView &bc, const S *__restrict__ src,
+                                                const typename Io<S>::C *__restrict__ side_prev,
+                                                const Geom &g, uint32_t n, uint32_t y,
+                                                typename Io<S>::C v[9]) {
+    using C = typename Io<S>::C;
+    const uint32_t code = bc.codes[n];
+    if (code == 0) return 0;
+    const ClassEntry &ce = bc.classes[code];
+    const uint32_t w = ce.src;
+#pragma unroll
+    for (int d = 0; d < 9; d++) {
+        const uint32_t kind = (w >> (2 * d)) & 3u;
+        if (kind == kSrcReflect) {
+            v[d] = Io<S>::ld(src + (size_t)kOpp[d] * g.N + n) - (C)ce.k[d];
+        } else if (kind == kSrcConst) {
+            if constexpr (Io<S>::kDev) v[d] = (C)(ce.k[d] - kWd[d]);
+            else v[d] = ce.k[d];
+        } else if (kind == kSrcOutlet) {
+            v[d] = side_prev[(size_t)y * 3 + outlet_slot(d)];
+        }
+    }
+    return ce.flags;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 arithmetic in the reference's operation order (no contraction)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void moments_f64(const double f[9], double &rho, double &jx, double &jy) {
+#pragma clang fp contract(off)
+    rho = (((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7]))) + f[8];
+    jx = ((f[1] + f[5]) + f[8]) - ((f[3] + f[6]) + f[7]);
+    jy = ((f[2] + f[5]) + f[6]) - ((f[4] + f[7]) + f[8]);
+}
+
+__device__ __forceinline__ void velocity_f64(double rho, double jx, double jy, double &ux, double &uy) {
+    if (rho != 0.0) {
+        ux = jx / rho;
+        uy = jy / rho;
+    } else {
+        ux = 0.0;
+        uy = 0.0;
+    }
+}
+
+__device__ __forceinline__ void feq_f64(double rho, double ux, double uy, double e[9]) {
+#pragma clang fp contract(off)
+    const double mag = __builtin_sqrt(ux * ux + uy * uy);
+    const double usq = mag * mag;
+    // c_i . u with exact products and one rounding, as numpy's u @ c.T
+    const double cu[9] = {0.0, ux, uy, -ux, -uy, ux + uy, uy - ux, -(ux + uy), ux - uy};
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+        e[i] = (kWd[i] * rho) * (((1.0 + 3.0 * cu[i]) + 4.5 * (cu[i] * cu[i])) - 1.5 * usq);
+}
+
+// ---------------------------------------------------------------------------
+// Collision of one node: moments of the post-boundary populations v, the
+// equilibrium, BGK relaxation.  Outputs the relaxed populations, the
+// equilibrium (needed by the pressure fix) and the moments.
+// ---------------------------------------------------------------------------
+template <class S>
+struct NodeOut {
+    using C = typename Io<S>::C;
+    C s[9];
+    C e[9];
+    double rho, jx, jy;  // for probes and reductions
+    C ux, uy;
+};
+
+template <class S>
+__device__ __forceinline__ void collide_node(const typename Io<S>::C v[9], double omega, NodeOut<S> &o) {
+    if constexpr (!Io<S>::kDev) {
+#pragma clang fp contract(off)
+        double rho, jx, jy, ux, uy;
+        moments_f64(v, rho, jx, jy);
+        velocity_f64(rho, jx, jy, ux, uy);
+        feq_f64(rho, ux, uy, o.e);
+#pragma unroll
+        for (int i = 0; i < 9; i++) o.s[i] = v[i] + (o.e[i] - v[i]) * omega;
+        o.rho = rho;
+        o.jx = jx;
+        o.jy = jy;
+        o.ux = ux;
+        o.uy = uy;
+    } else {
+        const float om = (float)omega;
+        const float drho = (((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]))) + v[8];
+        const float rho = 1.0f + drho;
+        const float jx = ((v[1] - v[3]) + (v[5] - v[7])) + (v[8] - v[6]);
+        const float jy = ((v[2] - v[4]) + (v[5] - v[7])) + (v[6] - v[8]);
+        const float inv = rho != 0.0f ? 1.0f / rho : 0.0f;
+        const float ux = jx * inv, uy = jy * inv;
+        const float base = -1.5f * (ux * ux + uy * uy);
+        const float cu[9] = {0.0f, ux, uy, -ux, -uy, ux + uy, uy - ux, -(ux + uy), ux - uy};
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            // f_eq,i - w_i = w_i * (drho + rho*(3 cu + 4.5 cu^2 - 1.5 u^2))
+            const float t = cu[i] * (3.0f + 4.5f * cu[i]) + base;
+            o.e[i] = (float)kWd[i] * (drho + rho * t);
+            o.s[i] = v[i] + om * (o.e[i] - v[i]);
+        }
+        o.rho = 1.0 + (double)drho;
+        o.jx = jx;
+        o.jy = jy;
+        o.ux = ux;
+        o.uy = uy;
+    }
+}
+
+// Equilibrium at (rho_b, u) of the node, in the policy's representation.
+template <class S>
+__device__ __forceinline__ void feq_rep(double rho_b, const NodeOut<S> &o, typename Io<S>::C e[9]) {
+    if constexpr (!Io<S>::kDev) {
+        feq_f64(rho_b, o.ux, o.uy, e);
+    } else {
+        const float ux = o.ux, uy = o.uy;
+        const float r = (float)rho_b, dr = (float)(rho_b - 1.0);
+        const float base = -1.5f * (ux * ux + uy * uy);
+        const float cu[9] = {0.0f, ux, uy, -ux, -uy, ux + uy, uy - ux, -(ux + uy), ux - uy};
+#pragma unroll
+        for (int i = 0; i < 9; i++) e[i] = (float)kWd[i] * (dr + r * (cu[i] * (3.0f + 4.5f * cu[i]) + base));
+    }
+}
+
+// Store the relaxed node plus the boundary side effects of its class:
+// outlet buffer (x = lx-2) and the pre-streaming pressure fix
+// (boundary_conditions.py:337-344): the node at x = lx-2 writes
+// E(rho_in, u) + (s - feq) into column 0 for directions D1, the node at x = 1
+// writes E(rho_out, u) + (s - feq) into column lx-1 for directions D2.
+template <class S>
+__device__ __forceinline__ void store_node(S *__restrict__ dst, typename Io<S>::C *__restrict__ side_next,
+                                           const Geom &g, const BcView &bc, uint32_t n, uint32_t y,
+                                           uint32_t flags, const typename Io<S>::C v[9],
+                                           const NodeOut<S> &o, bool nt) {
+    using C = typename Io<S>::C;
+    const uint32_t skip = flags >> kSkipShift;
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+        if (!((skip >> i) & 1u)) Io<S>::st(dst + (size_t)i * g.N + n, o.s[i], nt);
+    if (flags & kFlagOutletSide) {
+        side_next[(size_t)y * 3 + 0] = v[3];
+        side_next[(size_t)y * 3 + 1] = v[6];
+        side_next[(size_t)y * 3 + 2] = v[7];
+    }
+    if (flags & (kFlagPressureIn | kFlagPressureOut)) {
+#pragma clang fp contract(off)
+        C e[9];
+        if (flags & kFlagPressureIn) {
+            feq_rep<S>(bc.rho_in, o, e);
+#pragma unroll
+            for (int i = 0; i < 9; i++)
+                if ((bc.d1 >> i) & 1u) Io<S>::st(dst + (size_t)i * g.N + y, e[i] + (o.s[i] - o.e[i]), false);
+        }
+        if (flags & kFlagPressureOut) {
+            feq_rep<S>(bc.rho_out, o, e);
+            const uint32_t m = (g.lx - 1) * g.ly + y;
+#pragma unroll
+            for (int i = 0; i < 9; i++)
+                if ((bc.d2 >> i) & 1u) Io<S>::st(dst + (size_t)i * g.N + m, e[i] + (o.s[i] - o.e[i]), false);
+        }
+    }
+}
+
+// Sum three doubles over the block in a fixed order; thread 0 writes out[0..2].
+__device__ __forceinline__ void block_sum3(double a, double b, double c, double *out) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        c += __shfl_xor(c, off, 64);
+    }
+    __shared__ double sh[kBlock / 64][3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[wave][0] = a;
+        sh[wave][1] = b;
+        sh[wave][2] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            s0 += sh[w][0];
+            s1 += sh[w][1];
+            s2 += sh[w][2];
+        }
+        out[0] = s0;
+        out[1] = s1;
+        out[2] = s2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the fused step
+// ---------------------------------------------------------------------------
+template <class S>
+struct FusedParams {
+    const S *src;
+    S *dst;
+    const typename Io<S>::C *side_prev;
+    typename Io<S>::C *side_next;
+    Geom g;
+    BcView bc;
+    double omega;
+    int64_t probe_cell;
+    double *probe_out;  // 3 doubles, or null
+    double *partials;   // 3 doubles per block, or null
+    int xcd;
+    int nt;
+};
+
+template <class S, int V, bool kProbe, bool kReduce>
+__global__ __launch_bounds__(kBlock) void k_fused(FusedParams<S> p) {
+    using C = typename Io<S>::C;
+    const uint32_t blk = p.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const Geom &g = p.g;
+    C v[V][9];
+    uint32_t node[V], ys[V];
+    bool live[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+        node[k] = (blk * V + k) * kBlock + threadIdx.x;
+        live[k] = node[k] < g.N;
+        ys[k] = 0;
+        if (live[k]) {
+            const uint32_t x = row_of(g, node[k]);
+            ys[k] = node[k] - x * g.ly;
+            pull<S>(p.src, g, node[k], x, ys[k], v[k]);
+        }
+    }
+    double acc0 = 0, acc1 = 0, acc2 = 0;
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+        if (!live[k]) continue;
+        uint32_t flags = 0;
+        if (p.bc.chunk) {
+            const uint32_t chunk = __builtin_amdgcn_readfirstlane(node[k] >> 6);
+            if (p.bc.chunk[chunk]) flags = apply_class<S>(p.bc, p.src, p.side_prev, g, node[k], ys[k], v[k]);
+        }
+        NodeOut<S> o;
+        collide_node<S>(v[k], p.omega, o);
+        store_node<S>(p.dst, p.side_next, g, p.bc, node[k], ys[k], flags, v[k], o, p.nt != 0);
+        if constexpr (kProbe) {
+            if ((int64_t)node[k] == p.probe_cell) {
+                p.probe_out[0] = o.rho;
+                p.probe_out[1] = (double)o.ux;
+                p.probe_out[2] = (double)o.uy;
+            }
+        }
+        if constexpr (kReduce) {
+            acc0 += o.rho;
+            acc1 += o.jx;
+            acc2 += o.jy;
+        }
+    }
+    if constexpr (kReduce) block_sum3(acc0, acc1, acc2, p.partials + 3 * (size_t)blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// first collision (from the AoS FP64 step input)
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_collide_first(const double *__restrict__ f,
+                                                          const double *__restrict__ rho,
+                                                          const double *__restrict__ u, S *__restrict__ dst,
+                                                          typename Io<S>::C *__restrict__ side, Geom g,
+                                                          BcView bc, double omega) {
+#pragma clang fp contract(off)
+    const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= g.N) return;
+    const uint32_t x = row_of(g, n), y = n - x * g.ly;
+    double fi[9], e[9], s[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) fi[i] = f[(size_t)n * 9 + i];
+    const double r = rho[n], ux = u[2 * (size_t)n], uy = u[2 * (size_t)n + 1];
+    feq_f64(r, ux, uy, e);
+#pragma unroll
+    for (int i = 0; i < 9; i++) s[i] = fi[i] + (e[i] - fi[i]) * omega;
+    const uint32_t flags = bc.codes ? bc.classes[bc.codes[n]].flags : 0;
+    const uint32_t skip = flags >> kSkipShift;
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+        if (!((skip >> i) & 1u)) Io<S>::st(dst + (size_t)i * g.N + n, rep<S>(s[i], i), false);
+    if (flags & kFlagOutletSide) {
+        side[(size_t)y * 3 + 0] = rep<S>(fi[3], 3);
+        side[(size_t)y * 3 + 1] = rep<S>(fi[6], 6);
+        side[(size_t)y * 3 + 2] = rep<S>(fi[7], 7);
+    }
+    if (flags & kFlagPressureIn) {
+        double ein[9];
+        feq_f64(bc.rho_in, ux, uy, ein);
+        for (int i = 0; i < 9; i++)
+            if ((bc.d1 >> i) & 1u) Io<S>::st(dst + (size_t)i * g.N + y, rep<S>(ein[i] + (s[i] - e[i]), i), false);
+    }
+    if (flags & kFlagPressureOut) {
+        double eout[9];
+        feq_f64(bc.rho_out, ux, uy, eout);
+        const uint32_t m = (g.lx - 1) * g.ly + y;
+        for (int i = 0; i < 9; i++)
+            if ((bc.d2 >> i) & 1u) Io<S>::st(dst + (size_t)i * g.N + m, rep<S>(eout[i] + (s[i] - e[i]), i), false);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stream + boundary + moments into the reference's AoS FP64 layout
+// ---------------------------------------------------------------------------
+template <class S>
+__device__ __forceinline__ void finalize_node(const S *__restrict__ src, const typename Io<S>::C *side_prev,
+                                              const Geom &g, const BcView &bc, uint32_t n, double f[9],
+                                              double &rho, double &jx, double &jy, double &ux, double &uy) {
+    using C = typename Io<S>::C;
+    const uint32_t x = row_of(g, n), y = n - x * g.ly;
+    C v[9];
+    pull<S>(src, g, n, x, y, v);
+    if (bc.codes) apply_class<S>(bc, src, side_prev, g, n, y, v);
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        if constexpr (Io<S>::kDev) f[i] = (double)v[i] + kWd[i];
+        else f[i] = v[i];
+    }
+    moments_f64(f, rho, jx, jy);
+    velocity_f64(rho, jx, jy, ux, uy);
+}
+
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_finalize(const S *__restrict__ src,
+                                                     const typename Io<S>::C *__restrict__ side_prev, Geom g,
+                                                     BcView bc, double *__restrict__ fo, double *__restrict__ ro,
+                                                     double *__restrict__ uo, double *__restrict__ partials) {
+    const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+    double a0 = 0, a1 = 0, a2 = 0;
+    if (n < g.N) {
+        double f[9], rho, jx, jy, ux, uy;
+        finalize_node<S>(src, side_prev, g, bc, n, f, rho, jx, jy, ux, uy);
+        if (fo)
+            for (int i = 0; i < 9; i++) fo[(size_t)n * 9 + i] = f[i];
+        if (ro) ro[n] = rho;
+        if (uo) {
+            uo[2 * (size_t)n] = ux;
+            uo[2 * (size_t)n + 1] = uy;
+        }
+        a0 = rho;
+        a1 = jx;
+        a2 = jy;
+    }
+    if (partials) block_sum3(a0, a1, a2, partials + 3 * (size_t)blockIdx.x);
+}
+
+template <class S>
+__global__ void k_probe_cell(const S *__restrict__ src, const typename Io<S>::C *__restrict__ side_prev, Geom g,
+                             BcView bc, uint32_t n, double *out) {
+    if constexpr (Io<S>::kDev) {
+        // the values the fused kernel records for this node
+        using C = typename Io<S>::C;
+        const uint32_t x = row_of(g, n), y = n - x * g.ly;
+        C v[9];
+        pull<S>(src, g, n, x, y, v);
+        if (bc.codes) apply_class<S>(bc, src, side_prev, g, n, y, v);
+        NodeOut<S> o;
+        collide_node<S>(v, 1.0, o);
+        out[0] = o.rho;
+        out[1] = (double)o.ux;
+        out[2] = (double)o.uy;
+    } else {
+        double f[9], rho, jx, jy, ux, uy;
+        finalize_node<S>(src, side_prev, g, bc, n, f, rho, jx, jy, ux, uy);
+        out[0] = rho;
+        out[1] = ux;
+        out[2] = uy;
+    }
+}
+
+__global__ void k_sum_partials(const double *__restrict__ partials, int nblocks, double *out) {
+    // one block: strided partial sums in a fixed order, then a fixed tree
+    double a = 0, b = 0, c = 0;
+    for (int j = threadIdx.x; j < nblocks; j += kBlock) {
+        a += partials[3 * (size_t)j];
+        b += partials[3 * (size_t)j + 1];
+        c += partials[3 * (size_t)j + 2];
+    }
+    block_sum3(a, b, c, out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_moments_sum_aos(const double *__restrict__ f, int64_t n,
+                                                            double *partials) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double a0 = 0, a1 = 0, a2 = 0;
+    if (c < n) {
+        double v[9];
+        for (int i = 0; i < 9; i++) v[i] = f[c * 9 + i];
+        moments_f64(v, a0, a1, a2);
+    }
+    block_sum3(a0, a1, a2, partials + 3 * (size_t)blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// stateless AoS kernels (drop-ins for the reference's numpy functions)
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void feq_t(T rho, T ux, T uy, T e[9]) {
+#pragma clang fp contract(off)
+    const T mag = sqrt(ux * ux + uy * uy);
+    const T usq = mag * mag;
+    const T cu[9] = {T(0), ux, uy, -ux, -uy, ux + uy, uy - ux, -(ux + uy), ux - uy};
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+        e[i] = ((T)kWd[i] * rho) * (((T(1) + T(3) * cu[i]) + T(4.5) * (cu[i] * cu[i])) - T(1.5) * usq);
+}
+
+template <class T>
+__global__ void k_stream_aos(const T *__restrict__ in, T *__restrict__ out, int nx, int ny) {
+    const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (n >= (int64_t)nx * ny) return;
+    const int x = (int)(n / ny), y = (int)(n - (int64_t)x * ny);
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const int sx = (x - kCX[i] + nx) % nx, sy = (y - kCY[i] + ny) % ny;
+        out[n * 9 + i] = in[((int64_t)sx * ny + sy) * 9 + i];
+    }
+}
+
+template <class T>
+__global__ void k_equilibrium_aos(const T *__restrict__ rho, const T *__restrict__ u, T *__restrict__ feq,
+                                  int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    T e[9];
+    feq_t<T>(rho[c], u[2 * c], u[2 * c + 1], e);
+    for (int i = 0; i < 9; i++) feq[c * 9 + i] = e[i];
+}
+
+template <class T>
+__global__ void k_density_aos(const T *__restrict__ f, T *__restrict__ rho, int64_t n) {
+#pragma clang fp contract(off)
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const T *a = f + c * 9;
+    rho[c] = (((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]))) + a[8];
+}
+
+template <class T>
+__global__ void k_velocity_aos(const T *__restrict__ rho, const T *__restrict__ f, T *__restrict__ u, int64_t n) {
+#pragma clang fp contract(off)
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const T *a = f + c * 9;
+    const T jx = ((a[1] + a[5]) + a[8]) - ((a[3] + a[6]) + a[7]);
+    const T jy = ((a[2] + a[5]) + a[6]) - ((a[4] + a[7]) + a[8]);
+    const T r = rho[c];
+    u[2 * c] = r != T(0) ? jx / r : T(0);
+    u[2 * c + 1] = r != T(0) ? jy / r : T(0);
+}
+
+template <class T>
+__global__ void k_collide_aos(const T *__restrict__ f, const T *__restrict__ rho, const T *__restrict__ u,
+                              T omega, T *__restrict__ out, int64_t n) {
+#pragma clang fp contract(off)
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    T e[9];
+    feq_t<T>(rho[c], u[2 * c], u[2 * c + 1], e);
+    for (int i = 0; i < 9; i++) out[c * 9 + i] = f[c * 9 + i] + (e[i] - f[c * 9 + i]) * omega;
+}
+
+__global__ void k_bc_apply(Geom g, BcView bc, const double *__restrict__ f_pre, double *__restrict__ f_post,
+                           const double *__restrict__ f_prev) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= bc.nspecial) return;
+    const uint32_t n = bc.special[j];
+    const ClassEntry &ce = bc.classes[bc.codes[n]];
+    const uint32_t x = n / g.ly, y = n - x * g.ly;
+#pragma unroll
+    for (int d = 0; d < 9; d++) {
+        const uint32_t kind = (ce.src >> (2 * d)) & 3u;
+        if (kind == kSrcReflect) f_post[(size_t)n * 9 + d] = f_pre[(size_t)n * 9 + kOpp[d]] - ce.k[d];
+        else if (kind == kSrcConst) f_post[(size_t)n * 9 + d] = ce.k[d];
+        else if (kind == kSrcOutlet) f_post[(size_t)n * 9 + d] = f_prev[((size_t)(g.lx - 2) * g.ly + y) * 9 + d];
+    }
+}
+
+__global__ void k_bc_pressure(Geom g, BcView bc, double *__restrict__ f, const double *__restrict__ rho,
+                              const double *__restrict__ u) {
+#pragma clang fp contract(off)
+    const uint32_t y = blockIdx.x * kBlock + threadIdx.x;
+    if (y >= g.ly) return;
+    const uint32_t a = (g.lx - 2) * g.ly + y, b = (1 % g.lx) * g.ly + y;
+    double ein[9], ea[9], eout[9], eb[9];
+    feq_f64(bc.rho_in, u[2 * (size_t)a], u[2 * (size_t)a + 1], ein);
+    feq_f64(rho[a], u[2 * (size_t)a], u[2 * (size_t)a + 1], ea);
+    for (int i = 0; i < 9; i++)
+        if ((bc.d1 >> i) & 1u) f[(size_t)y * 9 + i] = ein[i] + (f[(size_t)a * 9 + i] - ea[i]);
+    feq_f64(bc.rho_out, u[2 * (size_t)b], u[2 * (size_t)b + 1], eout);
+    feq_f64(rho[b], u[2 * (size_t)b], u[2 * (size_t)b + 1], eb);
+    const size_t m = (size_t)(g.lx - 1) * g.ly + y;
+    for (int i = 0; i < 9; i++)
+        if ((bc.d2 >> i) & 1u) f[m * 9 + i] = eout[i] + (f[(size_t)b * 9 + i] - eb[i]);
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <class S, int V>
+hipError_t fused_dispatch(const FusedParams<S> &p, unsigned grid, bool probe, bool reduce, hipStream_t s) {
+    if (probe && reduce) hipLaunchKernelGGL((k_fused<S, V, true, true>), dim3(grid), dim3(kBlock), 0, s, p);
+    else if (probe) hipLaunchKernelGGL((k_fused<S, V, true, false>), dim3(grid), dim3(kBlock), 0, s, p);
+    else if (reduce) hipLaunchKernelGGL((k_fused<S, V, false, true>), dim3(grid), dim3(kBlock), 0, s, p);
+    else hipLaunchKernelGGL((k_fused<S, V, false, false>), dim3(grid), dim3(kBlock), 0, s, p);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t fused_typed(const Geom &g, const StateView &st, int src, const BcView &bc, double omega,
+                       const ProbeView &pr, int64_t slot, double *partials, const LaunchCfg &cfg,
+                       hipStream_t s, int *nblocks) {
+    using C = typename Io<S>::C;
+    FusedParams<S> p;
+    p.src = static_cast<const S *>(st.plane[src]);
+    p.dst = static_cast<S *>(st.plane[src ^ 1]);
+    p.side_prev = static_cast<const C *>(st.side[src]);
+    p.side_next = static_cast<C *>(st.side[src ^ 1]);
+    p.g = g;
+    p.bc = bc;
+    p.omega = omega;
+    const bool probe = pr.cell >= 0 && slot >= 0 && pr.ring;
+    p.probe_cell = probe ? pr.cell : -1;
+    p.probe_out = probe ? pr.ring + 3 * (slot % pr.capacity) : nullptr;
+    p.partials = partials;
+    p.xcd = cfg.xcd_remap;
+    p.nt = cfg.nt_stores;
+    const int V = cfg.cells_per_thread;
+    const unsigned grid = (unsigned)(((int64_t)g.N + (int64_t)kBlock * V - 1) / ((int64_t)kBlock * V));
+    if (nblocks) *nblocks = (int)grid;
+    switch (V) {
+        case 1: return fused_dispatch<S, 1>(p, grid, probe, partials != nullptr, s);
+        case 2: return fused_dispatch<S, 2>(p, grid, probe, partials != nullptr, s);
+        case 4: return fused_dispatch<S, 4>(p, grid, probe, partials != nullptr, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+int fused_blocks(const Geom &g, const LaunchCfg &cfg) {
+    const int64_t per = (int64_t)kBlock * cfg.cells_per_thread;
+    return (int)(((int64_t)g.N + per - 1) / per);
+}
+
+int partial_rows_needed(const Geom &g, const LaunchCfg &cfg) {
+    const int a = fused_blocks(g, cfg);
+    const int b = (int)grid_for(g.N);
+    return a > b ? a : b;
+}
+
+hipError_t launch_fused(int storage, const Geom &g, const StateView &st, int src, const BcView &bc,
+                        double omega, const ProbeView &pr, int64_t slot, double *partials,
+                        const LaunchCfg &cfg, hipStream_t s, int *nblocks) {
+    switch (storage) {
+        case LBM_STORE_F64: return fused_typed<double>(g, st, src, bc, omega, pr, slot, partials, cfg, s, nblocks);
+        case LBM_STORE_F32: return fused_typed<float>(g, st, src, bc, omega, pr, slot, partials, cfg, s, nblocks);
+        case LBM_STORE_F16: return fused_typed<__half>(g, st, src, bc, omega, pr, slot, partials, cfg, s, nblocks);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_collide_first(int storage, const Geom &g, const double *f, const double *rho, const double *u,
+                                const StateView &st, int dst, const BcView &bc, double omega, hipStream_t s) {
+    const unsigned grid = grid_for(g.N);
+    switch (storage) {
+        case LBM_STORE_F64:
+            hipLaunchKernelGGL(k_collide_first<double>, dim3(grid), dim3(kBlock), 0, s, f, rho, u,
+                               static_cast<double *>(st.plane[dst]), static_cast<double *>(st.side[dst]), g, bc,
+                               omega);
+            break;
+        case LBM_STORE_F32:
+            hipLaunchKernelGGL(k_collide_first<float>, dim3(grid), dim3(kBlock), 0, s, f, rho, u,
+                               static_cast<float *>(st.plane[dst]), static_cast<float *>(st.side[dst]), g, bc,
+                               omega);
+            break;
+        case LBM_STORE_F16:
+            hipLaunchKernelGGL(k_collide_first<__half>, dim3(grid), dim3(kBlock), 0, s, f, rho, u,
+                               static_cast<__half *>(st.plane[dst]), static_cast<float *>(st.side[dst]), g, bc,
+                               omega);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(int storage, const Geom &g, const StateView &st, int src, const BcView &bc, double *f,
+                           double *rho, double *u, double *partials, hipStream_t s, int *nblocks) {
+    const unsigned grid = grid_for(g.N);
+    if (nblocks) *nblocks = (int)grid;
+    switch (storage) {
+        case LBM_STORE_F64:
+            hipLaunchKernelGGL(k_finalize<double>, dim3(grid), dim3(kBlock), 0, s,
+                               static_cast<const double *>(st.plane[src]), static_cast<const double *>(st.side[src]),
+                               g, bc, f, rho, u, partials);
+            break;
+        case LBM_STORE_F32:
+            hipLaunchKernelGGL(k_finalize<float>, dim3(grid), dim3(kBlock), 0, s,
+                               static_cast<const float *>(st.plane[src]), static_cast<const float *>(st.side[src]), g,
+                               bc, f, rho, u, partials);
+            break;
+        case LBM_STORE_F16:
+            hipLaunchKernelGGL(k_finalize<__half>, dim3(grid), dim3(kBlock), 0, s,
+                               static_cast<const __half *>(st.plane[src]), static_cast<const float *>(st.side[src]),
+                               g, bc, f, rho, u, partials);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_cell(int storage, const Geom &g, const StateView &st, int src, const BcView &bc,
+                             const ProbeView &pr, int64_t slot, hipStream_t s) {
+    if (pr.cell < 0 || !pr.ring || pr.capacity <= 0) return hipSuccess;
+    double *out = pr.ring + 3 * (slot % pr.capacity);
+    const uint32_t n = (uint32_t)pr.cell;
+    switch (storage) {
+        case LBM_STORE_F64:
+            hipLaunchKernelGGL(k_probe_cell<double>, dim3(1), dim3(1), 0, s,
+                               static_cast<const double *>(st.plane[src]), static_cast<const double *>(st.side[src]),
+                               g, bc, n, out);
+            break;
+        case LBM_STORE_F32:
+            hipLaunchKernelGGL(k_probe_cell<float>, dim3(1), dim3(1), 0, s,
+                               static_cast<const float *>(st.plane[src]), static_cast<const float *>(st.side[src]), g,
+                               bc, n, out);
+            break;
+        case LBM_STORE_F16:
+            hipLaunchKernelGGL(k_probe_cell<__half>, dim3(1), dim3(1), 0, s,
+                               static_cast<const __half *>(st.plane[src]), static_cast<const float *>(st.side[src]),
+                               g, bc, n, out);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(const double *partials, int nblocks, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, partials, nblocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_moments_sum_aos(const double *f, int64_t n, double *partials, hipStream_t s, int *nblocks) {
+    const unsigned grid = grid_for(n);
+    if (nblocks) *nblocks = (int)grid;
+    hipLaunchKernelGGL(k_moments_sum_aos, dim3(grid), dim3(kBlock), 0, s, f, n, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_aos(const void *in, void *out, int nx, int ny, int dtype, hipStream_t s) {
+    const unsigned grid = grid_for((int64_t)nx * ny);
+    if (dtype == LBM_DT_F64)
+        hipLaunchKernelGGL(k_stream_aos<double>, dim3(grid), dim3(kBlock), 0, s, static_cast<const double *>(in),
+                           static_cast<double *>(out), nx, ny);
+    else
+        hipLaunchKernelGGL(k_stream_aos<float>, dim3(grid), dim3(kBlock), 0, s, static_cast<const float *>(in),
+                           static_cast<float *>(out), nx, ny);
+    return hipGetLastError();
+}
+
+hipError_t launch_equilibrium_aos(const void *rho, const void *u, void *feq, int64_t n, int dtype, hipStream_t s) {
+    if (dtype == LBM_DT_F64)
+        hipLaunchKernelGGL(k_equilibrium_aos<double>, dim3(grid_for(n)), dim3(kBlock), 0, s,
+                           static_cast<const double *>(rho), static_cast<const double *>(u),
+                           static_cast<double *>(feq), n);
+    else
+        hipLaunchKernelGGL(k_equilibrium_aos<float>, dim3(grid_for(n)), dim3(kBlock), 0, s,
+                           static_cast<const float *>(rho), static_cast<const float *>(u), static_cast<float *>(feq),
+                           n);
+    return hipGetLastError();
+}
+
+hipError_t launch_density_aos(const void *f, void *rho, int64_t n, int dtype, hipStream_t s) {
+    if (dtype == LBM_DT_F64)
+        hipLaunchKernelGGL(k_density_aos<double>, dim3(grid_for(n)), dim3(kBlock), 0, s,
+                           static_cast<const double *>(f), static_cast<double *>(rho), n);
+    else
+        hipLaunchKernelGGL(k_density_aos<float>, dim3(grid_for(n)), dim3(kBlock), 0, s,
+                           static_cast<const float *>(f), static_cast<float *>(rho), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_velocity_aos(const void *rho, const void *f, void *u, int64_t n, int dtype, hipStream_t s) {
+    if (dtype == LBM_DT_F64)
+        hipLaunchKernelGGL(k_velocity_aos<double>, dim3(grid_for(n)), dim3(kBlock), 0, s,
+                           static_cast<const double *>(rho), static_cast<const double *>(f),
+                           static_cast<double *>(u), n);
+    else
+        hipLaunchKernelGGL(k_velocity_aos<float>, dim3(grid_for(n)), dim3(kBlock), 0, s,
+                           static_cast<const float *>(rho), static_cast<const float *>(f), static_cast<float *>(u),
+                           n);
+    return hipGetLastError();
+}
+
+hipError_t launch_collide_aos(const void *f, const void *rho, const void *u, double omega, void *out, int64_t n,
+                              int dtype, hipStream_t s) {
+    if (dtype == LBM_DT_F64)
+        hipLaunchKernelGGL(k_collide_aos<double>, dim3(grid_for(n)), dim3(kBlock), 0, s,
+                           static_cast<const double *>(f), static_cast<const double *>(rho),
+                           static_cast<const double *>(u), omega, static_cast<double *>(out), n);
+    else
+        hipLaunchKernelGGL(k_collide_aos<float>, dim3(grid_for(n)), dim3(kBlock), 0, s,
+                           static_cast<const float *>(f), static_cast<const float *>(rho),
+                           static_cast<const float *>(u), (float)omega, static_cast<float *>(out), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc_apply(const Geom &g, const BcView &bc, const double *f_pre, double *f_post, const double *f_prev,
+                           hipStream_t s) {
+    if (bc.nspecial == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bc_apply, dim3(grid_for(bc.nspecial)), dim3(kBlock), 0, s, g, bc, f_pre, f_post, f_prev);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc_pressure(const Geom &g, const BcView &bc, double *f_pre, const double *rho, const double *u,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_pressure, dim3(grid_for(g.ly)), dim3(kBlock), 0, s, g, bc, f_pre, rho, u);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

@@ -1,0 +1,93 @@
+// Kernel argument blocks and host-side launch wrappers (lbm_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lbm_internal.h"
+
+namespace lbm {
+
+// Boundary tables as the kernels see them (device pointers).
+struct BcView {
+    const uint16_t *codes = nullptr;       // class per node, N
+    const uint8_t *chunk = nullptr;        // nonzero if any node of a 64-node chunk is special
+    const ClassEntry *classes = nullptr;
+    const uint32_t *special = nullptr;     // list of special nodes
+    int64_t nspecial = 0;
+    double rho_in = 0, rho_out = 0;        // pressure fix
+    uint32_t d1 = 0, d2 = 0;
+    uint32_t flags = 0;                    // kProg*
+};
+
+// Geometry with a precomputed divisor for node -> (x, y).
+struct Geom {
+    uint32_t lx = 0, ly = 0, N = 0;
+    uint64_t ly_magic = 0;  // floor(2^64 / ly) + 1 (exact 32-bit division by ly)
+};
+
+Geom make_geom(uint32_t lx, uint32_t ly);
+
+// The state buffers of one lattice: nine planes (of the storage type) and the
+// outlet side buffers (3*ly of the compute type), each double-buffered.
+struct StateView {
+    void *plane[2] = {nullptr, nullptr};
+    void *side[2] = {nullptr, nullptr};
+};
+
+struct ProbeView {
+    int64_t cell = -1;       // node index, -1 = off
+    double *ring = nullptr;  // capacity * 3 doubles (rho, ux, uy)
+    int64_t capacity = 0;
+};
+
+struct LaunchCfg {
+    int cells_per_thread = 2;
+    int xcd_remap = 1;
+    int nt_stores = 0;
+};
+
+// One fused step: planes[src] -> planes[dst], side[src] -> side[dst].
+// probe_slot < 0: no probe sample; partials != nullptr: per-block moment sums.
+hipError_t launch_fused(int storage, const Geom &g, const StateView &st, int src, const BcView &bc,
+                        double omega, const ProbeView &pr, int64_t probe_slot, double *partials,
+                        const LaunchCfg &cfg, hipStream_t s, int *nblocks_out);
+int fused_blocks(const Geom &g, const LaunchCfg &cfg);
+
+// First collision from AoS FP64 (f, rho, u) into planes[dst] / side[dst].
+hipError_t launch_collide_first(int storage, const Geom &g, const double *f, const double *rho,
+                                const double *u, const StateView &st, int dst, const BcView &bc,
+                                double omega, hipStream_t s);
+
+// Stream + boundary + moments from planes[src] into AoS FP64 outputs (any may
+// be null); partials != nullptr adds the per-block moment sums.
+hipError_t launch_finalize(int storage, const Geom &g, const StateView &st, int src, const BcView &bc,
+                           double *f, double *rho, double *u, double *partials, hipStream_t s,
+                           int *nblocks_out);
+
+// Moments of one node into probe.ring[slot].
+hipError_t launch_probe_cell(int storage, const Geom &g, const StateView &st, int src, const BcView &bc,
+                             const ProbeView &pr, int64_t slot, hipStream_t s);
+
+// Sum nblocks rows of 3 partials into out[3] in a fixed order.
+hipError_t launch_sum_partials(const double *partials, int nblocks, double *out, hipStream_t s);
+// Moment sums of AoS FP64 populations (nblocks_out partial rows).
+hipError_t launch_moments_sum_aos(const double *f, int64_t n, double *partials, hipStream_t s,
+                                  int *nblocks_out);
+int partial_rows_needed(const Geom &g, const LaunchCfg &cfg);
+
+// Stateless AoS kernels; dtype LBM_DT_F64 / LBM_DT_F32.
+hipError_t launch_stream_aos(const void *in, void *out, int nx, int ny, int dtype, hipStream_t s);
+hipError_t launch_equilibrium_aos(const void *rho, const void *u, void *feq, int64_t n, int dtype,
+                                  hipStream_t s);
+hipError_t launch_density_aos(const void *f, void *rho, int64_t n, int dtype, hipStream_t s);
+hipError_t launch_velocity_aos(const void *rho, const void *f, void *u, int64_t n, int dtype,
+                               hipStream_t s);
+hipError_t launch_collide_aos(const void *f, const void *rho, const void *u, double omega, void *out,
+                              int64_t n, int dtype, hipStream_t s);
+hipError_t launch_bc_apply(const Geom &g, const BcView &bc, const double *f_pre, double *f_post,
+                           const double *f_prev, hipStream_t s);
+hipError_t launch_bc_pressure(const Geom &g, const BcView &bc, double *f_pre, const double *rho,
+                              const double *u, hipStream_t s);
+
+}  // namespace lbm

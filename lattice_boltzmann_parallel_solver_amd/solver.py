@@ -1,0 +1,259 @@
+"""Device-resident lattice: the performance API over the C ABI.
+
+``Lattice`` keeps one D2Q9 lattice on one GPU and advances it with the fused
+stream+collide kernel.  Its state semantics are those of the reference's
+``lattice_boltzmann_step`` loop (lattice_boltzmann_method.py:191-228):
+``set_state(f, rho, u)`` loads the step input, ``run(omega, n)`` performs n
+reference steps, ``state()`` returns (f_n, rho_n, u_n).
+
+``BoundaryProgram`` is a compiled, ordered list of boundary descriptors
+(boundary_conditions.py / boundary_utils.py semantics, last writer wins).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Iterable, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+_c_p = ctypes.c_void_p
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _is_torch(x) -> bool:
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return False
+    return isinstance(x, torch.Tensor)
+
+
+def _stream_of_torch() -> int:
+    import torch
+
+    return torch.cuda.current_stream().cuda_stream
+
+
+class BoundaryProgram:
+    """Compiled boundary descriptors for an lx x ly lattice (lbm_bc_compile)."""
+
+    def __init__(self, lx: int, ly: int, descriptors: Sequence[dict]):
+        L = _lib.load()
+        self.lx, self.ly = int(lx), int(ly)
+        n = len(descriptors)
+        arr = (_lib.BcDesc * max(1, n))()
+        keep = []
+        for j, d in enumerate(descriptors):
+            arr[j].kind = int(d["kind"])
+            arr[j].reserved = 0
+            mask = d.get("mask")
+            if mask is not None:
+                m = np.ascontiguousarray(np.asarray(mask) != 0, dtype=np.uint8)
+                if m.shape != (self.lx, self.ly):
+                    raise ValueError(f"mask shape {m.shape} does not match lattice {(self.lx, self.ly)}")
+                keep.append(m)
+                arr[j].mask = m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+            uw = d.get("u_w", (0.0, 0.0))
+            arr[j].u_w[0] = float(uw[0])
+            arr[j].u_w[1] = float(uw[1])
+            arr[j].rho_w = float(d.get("rho_w", 0.0))
+            arr[j].rho_in = float(d.get("rho_in", 0.0))
+            arr[j].u_in = float(d.get("u_in", 0.0))
+            arr[j].p_in = float(d.get("p_in", 0.0))
+            arr[j].p_out = float(d.get("p_out", 0.0))
+        h = _c_p()
+        check(L.lbm_bc_compile(ctypes.byref(h), self.lx, self.ly, arr, n))
+        self._h = h
+        del keep
+
+    @property
+    def handle(self):
+        return self._h
+
+    def query(self) -> Tuple[int, int, int]:
+        nc = ctypes.c_int()
+        ns = ctypes.c_int64()
+        fl = ctypes.c_uint32()
+        check(_lib.load().lbm_bc_query(self._h, ctypes.byref(nc), ctypes.byref(ns), ctypes.byref(fl)))
+        return nc.value, ns.value, fl.value
+
+    @property
+    def has_outlet(self) -> bool:
+        return bool(self.query()[2] & 1)
+
+    @property
+    def has_pressure(self) -> bool:
+        return bool(self.query()[2] & 2)
+
+    def export(self) -> dict:
+        """Compiled tables: codes (lx, ly), per-class src/flags/k, pressure params."""
+        nc, _, _ = self.query()
+        codes = np.zeros((self.lx, self.ly), dtype=np.uint16)
+        src = np.zeros(nc, dtype=np.uint32)
+        flags = np.zeros(nc, dtype=np.uint32)
+        k = np.zeros((nc, 9), dtype=np.float64)
+        params = np.zeros(4, dtype=np.float64)
+        check(_lib.load().lbm_bc_export(self._h, _ptr(codes), _ptr(src), _ptr(flags), _ptr(k), _ptr(params)))
+        return {"codes": codes, "src": src, "flags": flags, "k": k, "params": params}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().lbm_bc_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class Lattice:
+    """One lattice resident on one GPU (lbm_create / lbm_run / lbm_get_state).
+
+    storage: "f64" (FP64 populations, reference arithmetic order), "f32"
+    (FP32 deviations f - w), "f16" (FP16 deviations, FP32 arithmetic).
+    boundary: a BoundaryProgram, an object with ``program(lx, ly)``, or None
+    (fully periodic).
+    """
+
+    def __init__(self, lx: int, ly: int, boundary=None, storage: str = "f64", device: int = 0,
+                 stream: Optional[int] = None):
+        L = _lib.load()
+        if storage not in _lib.STORAGE:
+            raise ValueError(f"storage must be one of {sorted(_lib.STORAGE)}")
+        self.lx, self.ly, self.storage = int(lx), int(ly), storage
+        h = _c_p()
+        check(L.lbm_create(ctypes.byref(h), self.lx, self.ly, _lib.STORAGE[storage], int(device)))
+        self._h = h
+        self.device = int(device)
+        if stream is not None:
+            check(L.lbm_set_stream(h, _c_p(stream)))
+        self._program = None
+        self.set_boundary(boundary)
+
+    # -- configuration --------------------------------------------------------
+    def set_boundary(self, boundary) -> None:
+        if boundary is None:
+            prog = None
+        elif isinstance(boundary, BoundaryProgram):
+            prog = boundary
+        elif hasattr(boundary, "program"):
+            prog = boundary.program(self.lx, self.ly)
+        else:
+            raise TypeError("boundary must be a BoundaryProgram or carry .program(lx, ly)")
+        check(_lib.load().lbm_set_bc(self._h, prog.handle if prog is not None else None))
+        self._program = prog
+
+    def set_launch(self, cells_per_thread: int = 2, xcd_remap: bool = True, nt_stores: bool = False) -> None:
+        check(_lib.load().lbm_set_launch(self._h, int(cells_per_thread), int(bool(xcd_remap)), int(bool(nt_stores))))
+
+    # -- state ------------------------------------------------------------------
+    def _shape_check(self, f=None, rho=None, u=None):
+        if f is not None:
+            assert tuple(f.shape) == (self.lx, self.ly, 9), f"f must have shape {(self.lx, self.ly, 9)}"
+        if rho is not None:
+            assert tuple(rho.shape) == (self.lx, self.ly), f"density must have shape {(self.lx, self.ly)}"
+        if u is not None:
+            assert tuple(u.shape) == (self.lx, self.ly, 2), f"velocity must have shape {(self.lx, self.ly, 2)}"
+
+    def set_state(self, f, rho, u) -> None:
+        """Load the step input (f, rho, u); numpy arrays or float64 device tensors."""
+        self._shape_check(f, rho, u)
+        L = _lib.load()
+        if _is_torch(f):
+            import torch
+
+            ts = [t.detach().to(dtype=torch.float64).contiguous() for t in (f, rho, u)]
+            for t in ts:
+                if t.device.type != "cuda":
+                    raise ValueError("torch inputs must be device tensors")
+            torch.cuda.current_stream().synchronize()
+            check(L.lbm_set_state(self._h, ts[0].data_ptr(), ts[1].data_ptr(), ts[2].data_ptr(), _lib.DEVICE))
+        else:
+            a = [np.ascontiguousarray(x, dtype=np.float64) for x in (f, rho, u)]
+            check(L.lbm_set_state(self._h, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _lib.HOST))
+
+    def set_equilibrium(self, rho, u) -> None:
+        """Load f = equilibrium(rho, u) with (rho, u) as the step input."""
+        self._shape_check(None, rho, u)
+        L = _lib.load()
+        if _is_torch(rho):
+            import torch
+
+            ts = [t.detach().to(dtype=torch.float64).contiguous() for t in (rho, u)]
+            torch.cuda.current_stream().synchronize()
+            check(L.lbm_set_state_equilibrium(self._h, ts[0].data_ptr(), ts[1].data_ptr(), _lib.DEVICE))
+        else:
+            a = [np.ascontiguousarray(x, dtype=np.float64) for x in (rho, u)]
+            check(L.lbm_set_state_equilibrium(self._h, _ptr(a[0]), _ptr(a[1]), _lib.HOST))
+
+    def run(self, omega: float, steps: int) -> None:
+        """Advance `steps` reference steps (asynchronous)."""
+        check(_lib.load().lbm_run(self._h, float(omega), int(steps)))
+
+    def state(self, want=("f", "rho", "u")):
+        """(f, rho, u) of the current step as float64 numpy arrays."""
+        f = np.empty((self.lx, self.ly, 9)) if "f" in want else None
+        rho = np.empty((self.lx, self.ly)) if "rho" in want else None
+        u = np.empty((self.lx, self.ly, 2)) if "u" in want else None
+        check(_lib.load().lbm_get_state(self._h, None if f is None else _ptr(f), None if rho is None else _ptr(rho),
+                                        None if u is None else _ptr(u), _lib.HOST))
+        return f, rho, u
+
+    def state_into(self, f=None, rho=None, u=None) -> None:
+        """Write (f, rho, u) into float64 device tensors (any may be None)."""
+        import torch
+
+        for t in (f, rho, u):
+            if t is not None and (t.dtype != torch.float64 or not t.is_contiguous() or t.device.type != "cuda"):
+                raise ValueError("outputs must be contiguous float64 device tensors")
+        self._shape_check(f, rho, u)
+        torch.cuda.current_stream().synchronize()
+        check(_lib.load().lbm_get_state(self._h, None if f is None else f.data_ptr(),
+                                        None if rho is None else rho.data_ptr(),
+                                        None if u is None else u.data_ptr(), _lib.DEVICE))
+
+    # -- diagnostics --------------------------------------------------------------
+    def set_probe(self, x: int, y: int, capacity: int) -> None:
+        check(_lib.load().lbm_set_probe(self._h, int(x), int(y), int(capacity)))
+
+    def probe(self, max_rows: Optional[int] = None) -> np.ndarray:
+        """Recorded (rho, ux, uy) rows, oldest first."""
+        n = self.steps if max_rows is None else int(max_rows)
+        out = np.zeros((max(1, n), 3))
+        cnt = ctypes.c_int64()
+        check(_lib.load().lbm_read_probe(self._h, _ptr(out), n, ctypes.byref(cnt)))
+        return out[:cnt.value]
+
+    def moments_sum(self) -> Tuple[float, float, float]:
+        """FP64 (sum rho, sum rho*ux, sum rho*uy) of the current state."""
+        out = (ctypes.c_double * 3)()
+        check(_lib.load().lbm_moments_sum(self._h, out))
+        return out[0], out[1], out[2]
+
+    def synchronize(self) -> None:
+        check(_lib.load().lbm_synchronize(self._h))
+
+    @property
+    def steps(self) -> int:
+        s = ctypes.c_int64()
+        check(_lib.load().lbm_steps_done(self._h, ctypes.byref(s)))
+        return s.value
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.load().lbm_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,496 @@
+"""CPU FP64 oracle for the D2Q9 lattice-Boltzmann step (numpy restatement).
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: it is imported by
+``tests/``, by ``__graft_entry__.smoke()`` and by the ``cpu_baseline`` leg of
+``bench.py``.  The product package (``lattice_boltzmann_parallel_solver_amd``)
+never imports it, and nothing here is on the GPU path.
+
+What it restates
+----------------
+The reference solver is numpy code; this module re-derives the same arithmetic
+from the reference's definitions, operation by operation, so that its FP64
+results are bit-identical to the reference's where numpy is correctly rounded:
+
+* D2Q9 tables                      ``src/lattice_boltzmann_method.py:5-52``
+* density   rho = sum_i f_i        ``src/lattice_boltzmann_method.py:93-105``
+  numpy reduces a contiguous axis of length 9 pairwise:
+  ``(((f0+f1)+(f2+f3))+((f4+f5)+(f6+f7)))+f8``.
+* velocity  u = (sum c_i f_i)/rho  ``src/lattice_boltzmann_method.py:108-137``
+  ``ux = ((f1+f5)+f8) - ((f3+f6)+f7)``, ``uy = ((f2+f5)+f6) - ((f4+f7)+f8)``,
+  divided by rho where rho != 0, else 0.
+* streaming (periodic shift)       ``src/lattice_boltzmann_method.py:140-159``
+* equilibrium                      ``src/lattice_boltzmann_method.py:162-188``
+  ``feq_i = (w_i*rho) * (((1 + 3*cu) + 4.5*cu^2) - 1.5*|u|^2)`` with
+  ``cu = c_i . u`` (exact products, one rounding) and
+  ``|u|^2 = sqrt(ux*ux + uy*uy)^2`` (``np.linalg.norm`` then ``np.power``).
+* BGK step                         ``src/lattice_boltzmann_method.py:191-228``
+* boundary primitives              ``src/boundary_conditions.py:8-348``
+* boundary composites              ``src/boundary_utils.py:9-205``
+  (single-domain semantics; the MPI index mapping is not restated)
+* initial states                   ``src/initial_values.py:38-123``
+* Strouhal FFT procedure           ``src/visualizations_utils.py:149-167``
+* Reynolds / Strouhal numbers      ``src/lattice_boltzmann_method.py:55-90``
+
+Squaring semantics.  ``np.power(x, 2)`` is ``x*x`` in numpy 2.x, but numpy
+builds that route it through libm ``pow`` round differently in about 0.1 % of
+cases.  ``SQUARE`` selects the variant ("mul" by default, "pow" for libm).
+
+How each part is pinned (tests/test_oracle_*.py)
+-----------------------------------------------
+* tables, streaming: closed form; streaming is a permutation (bit-exact).
+* density/velocity/equilibrium: numpy's own reductions on random inputs
+  (rounding order), invariants sum feq = rho and sum c feq = rho u, and golden
+  vectors written by ``tests/golden/make_golden.py`` from the reference.
+* BC primitives: the reference's known-answer tests
+  (``tests/test_boundary_conditions.py:12-121``) plus the pressure constants
+  re-derived by hand.
+* step + composites: golden multi-step trajectories from the reference, the
+  12-value probe fixture ``tests/von_karman_vortex_shedding/vel_at_p.npy``,
+  and the reference's published Couette / Poiseuille fits
+  (``figures/couette_flow/linregress.csv``,
+  ``figures/poiseuille_flow/areas.csv``).
+* shear-wave decay: nu = (1/omega - 1/2)/3 (closed form).
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Callable, Optional, Sequence, Tuple
+
+import numpy as np
+
+# --------------------------------------------------------------------------
+# D2Q9 tables (src/lattice_boltzmann_method.py:5-52)
+# 0 rest, 1 E, 2 N, 3 W, 4 S, 5 NE, 6 NW, 7 SW, 8 SE
+# --------------------------------------------------------------------------
+CX = (0, 1, 0, -1, 0, 1, -1, -1, 1)
+CY = (0, 0, 1, 0, -1, 1, 1, -1, -1)
+C = np.array(list(zip(CX, CY)), dtype=np.int64)
+OPP = np.array([0, 3, 4, 1, 2, 7, 8, 5, 6], dtype=np.int64)
+W = np.array([4 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 36, 1 / 36, 1 / 36, 1 / 36])
+# speed of sound squared exactly as the reference forms it: (1/sqrt(3))**2
+CS2 = float((1 / np.sqrt(3)) ** 2)
+
+SQUARE = "mul"
+
+
+def _sq(x):
+    """x**2 in the selected rounding semantics."""
+    if SQUARE == "mul":
+        return x * x
+    if SQUARE == "pow":
+        return np.frompyfunc(lambda v: math.pow(v, 2.0), 1, 1)(x).astype(np.float64)
+    raise ValueError(SQUARE)
+
+
+# --------------------------------------------------------------------------
+# moments, equilibrium, collision, streaming
+# --------------------------------------------------------------------------
+def density(f: np.ndarray) -> np.ndarray:
+    """rho = sum_i f_i in numpy's pairwise order (lattice_boltzmann_method.py:93-105)."""
+    assert f.shape[-1] == 9
+    a = [f[..., i] for i in range(9)]
+    return (((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]))) + a[8]
+
+
+def velocity(rho: np.ndarray, f: np.ndarray) -> np.ndarray:
+    """u = (sum_i c_i f_i)/rho, zero where rho == 0 (lattice_boltzmann_method.py:108-137).
+
+    The caller's rho is used as given; it is not recomputed from f.
+    """
+    assert f.shape[-1] == 9
+    jx = ((f[..., 1] + f[..., 5]) + f[..., 8]) - ((f[..., 3] + f[..., 6]) + f[..., 7])
+    jy = ((f[..., 2] + f[..., 5]) + f[..., 6]) - ((f[..., 4] + f[..., 7]) + f[..., 8])
+    nz = rho != 0
+    safe = np.where(nz, rho, 1.0)
+    ux = np.where(nz, jx / safe, 0.0)
+    uy = np.where(nz, jy / safe, 0.0)
+    return np.stack([ux, uy], axis=-1)
+
+
+def equilibrium(rho: np.ndarray, u: np.ndarray) -> np.ndarray:
+    """Second-order equilibrium (lattice_boltzmann_method.py:162-188), any leading shape."""
+    rho = np.asarray(rho, dtype=np.float64)
+    u = np.asarray(u, dtype=np.float64)
+    assert rho.shape == u.shape[:-1]
+    ux = u[..., 0]
+    uy = u[..., 1]
+    usq = _sq(np.sqrt(ux * ux + uy * uy))
+    out = np.empty(rho.shape + (9,), dtype=np.float64)
+    for i in range(9):
+        # c_i . u: products with 0/+-1 are exact, the sum rounds once
+        cu = ux * float(CX[i]) + uy * float(CY[i])
+        out[..., i] = (W[i] * rho) * (((1.0 + 3.0 * cu) + 4.5 * _sq(cu)) - 1.5 * usq)
+    return out
+
+
+def collide(f: np.ndarray, feq: np.ndarray, omega: float) -> np.ndarray:
+    """BGK relaxation f + (feq - f)*omega (lattice_boltzmann_method.py:215)."""
+    return f + (feq - f) * omega
+
+
+def streaming(f: np.ndarray) -> np.ndarray:
+    """Periodic pull: out[x, y, i] = f[(x - cx_i) % lx, (y - cy_i) % ly, i].
+
+    Same permutation as the per-channel roll of lattice_boltzmann_method.py:140-159.
+    """
+    assert f.shape[-1] == 9
+    lx, ly = f.shape[0], f.shape[1]
+    out = np.empty_like(f)
+    xs = np.arange(lx)
+    ys = np.arange(ly)
+    for i in range(9):
+        sx = (xs - CX[i]) % lx
+        sy = (ys - CY[i]) % ly
+        out[:, :, i] = f[sx[:, None], sy[None, :], i]
+    return out
+
+
+def step(f: np.ndarray, rho: np.ndarray, u: np.ndarray, omega: float,
+         boundary: Optional[Callable] = None) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """One reference step (lattice_boltzmann_method.py:191-228).
+
+    collide with the *passed* rho,u -> stream -> boundary(f_pre, f_post, rho, u, f)
+    -> moments of the post-boundary populations.
+    """
+    assert f.shape[0:2] == rho.shape
+    assert f.shape[0:2] == u.shape[0:2]
+    assert 0 < omega < 2
+    f_pre = collide(f, equilibrium(rho, u), omega)
+    f_post = streaming(f_pre)
+    if boundary is not None:
+        f_post = boundary(f_pre, f_post, rho, u, f)
+    rho_new = density(f_post)
+    u_new = velocity(rho_new, f_post)
+    return f_post, rho_new, u_new
+
+
+# --------------------------------------------------------------------------
+# boundary primitives (src/boundary_conditions.py)
+# --------------------------------------------------------------------------
+def wall_directions(mask: np.ndarray) -> Tuple[int, ...]:
+    """Directions reflected by a straight wall (boundary_conditions.py:8-28).
+
+    The first full edge found decides: column x=0 -> (1,5,8), column x=-1 ->
+    (3,6,7), row y=0 -> (4,7,8), row y=-1 -> (2,5,6); otherwise nothing.
+    """
+    if np.all(mask[0, :]):
+        return (1, 5, 8)
+    if np.all(mask[-1, :]):
+        return (3, 6, 7)
+    if np.all(mask[:, 0]):
+        return (4, 7, 8)
+    if np.all(mask[:, -1]):
+        return (2, 5, 6)
+    return ()
+
+
+def plate_corners(mask: np.ndarray):
+    """Corner cells of an obstacle and the directions each reflects (boundary_conditions.py:31-56)."""
+    assert mask.ndim == 2
+    idx = np.argwhere(mask)
+    x0, x1 = int(np.amin(idx[:, 0])), int(np.amax(idx[:, 0]))
+    y0, y1 = int(np.amin(idx[:, 1])), int(np.amax(idx[:, 1]))
+    return [((x0, y1), (1, 8)),       # upper left
+            ((x0, y0), (1, 5)),       # lower left
+            ((x1 + 1, y1), (3, 7)),   # upper right
+            ((x1 + 1, y0), (3, 6))]   # lower right
+
+
+def rigid_wall(mask: np.ndarray):
+    """Same-node bounce-back on a straight wall (boundary_conditions.py:78-113)."""
+    assert mask.dtype == bool
+    dirs = wall_directions(mask)
+
+    def bc(f_pre, f_post):
+        assert mask.shape == f_pre.shape[0:2]
+        assert mask.shape == f_post.shape[0:2]
+        for i in dirs:
+            f_post[mask, OPP[i]] = f_pre[mask, i]
+        return f_post
+
+    bc.oracle_kind = ("reflect", [(mask.copy(), dirs, (0.0,) * 9)])
+    return bc
+
+
+def moving_wall(mask: np.ndarray, u_w, rho_w):
+    """Bounce-back with wall momentum (boundary_conditions.py:170-214).
+
+    f_post[b, opp(i)] = f_pre[b, i] - ((2*w_i)*rho_w) * ((c_i . u_w)/cs^2)
+    """
+    assert mask.dtype == bool
+    dirs = wall_directions(mask)
+    u_w = np.asarray(u_w)
+    k = [0.0] * 9
+    for i in dirs:
+        dot = float(C[i] @ u_w)
+        k[i] = float((2 * W[i]) * rho_w * np.divide(dot, CS2))
+
+    def bc(f_pre, f_post):
+        assert mask.shape == f_pre.shape[0:2]
+        assert mask.shape == f_post.shape[0:2]
+        for i in dirs:
+            f_post[mask, OPP[i]] = f_pre[mask, i] - k[i]
+        return f_post
+
+    bc.oracle_kind = ("reflect", [(mask.copy(), dirs, tuple(k))])
+    return bc
+
+
+def rigid_object(mask: np.ndarray):
+    """Bounce-back on a zero-thickness obstacle (boundary_conditions.py:116-167).
+
+    Like the reference this clears the corner cells in the caller's mask.
+    Left faces reflect (1,5,8) on the mask, right faces reflect (3,6,7) on the
+    mask shifted one column in +x, and each corner reflects its two directions.
+    """
+    assert mask.dtype == bool
+    corners = plate_corners(mask)
+    for (cx, cy), _ in corners:
+        mask[cx, cy] = False
+    left = mask
+    right = np.roll(mask, 1, axis=0)
+    rules = [(left.copy(), (1, 5, 8), (0.0,) * 9), (right.copy(), (3, 6, 7), (0.0,) * 9)]
+    for (cx, cy), dirs in corners:
+        m = np.zeros_like(mask)
+        m[cx, cy] = True
+        rules.append((m, dirs, (0.0,) * 9))
+
+    def bc(f_pre, f_post):
+        assert left.shape == f_pre.shape[0:2]
+        assert left.shape == f_post.shape[0:2]
+        for m, dirs, _ in rules:
+            for i in dirs:
+                f_post[m, OPP[i]] = f_pre[m, i]
+        return f_post
+
+    bc.oracle_kind = ("reflect", rules)
+    return bc
+
+
+def inlet(shape: Tuple[int, int], rho_in: float, u_in: float):
+    """Equilibrium inlet on column x=0, all nine directions (boundary_conditions.py:217-254)."""
+    vel = np.zeros(tuple(shape) + (2,))
+    vel[..., 0] = u_in
+    feq = equilibrium(np.ones(tuple(shape)) * rho_in, vel)
+
+    def bc(f_post):
+        f_post[0, :, :] = feq[0, :, :]
+        return f_post
+
+    bc.oracle_kind = ("inlet", feq[0, 0, :].copy())
+    return bc
+
+
+def outlet():
+    """Copy directions (3,6,7) from column x=-2 of the step-input f (boundary_conditions.py:257-283)."""
+    def bc(f_prev, f_post):
+        for i in (3, 6, 7):
+            f_post[-1, :, i] = f_prev[-2, :, i]
+        return f_post
+
+    bc.oracle_kind = ("outlet", None)
+    return bc
+
+
+def periodic_with_pressure_variations(mask: np.ndarray, p_in: float, p_out: float):
+    """Pre-streaming pressure fix on columns x=0 and x=-1 (boundary_conditions.py:286-348).
+
+    With f the post-collision populations, feq = E(rho, u) of the step-input
+    moments, rho_in = p_in/cs^2 and rho_out = p_out/cs^2:
+      f[0,  :, D1] = E(rho_in,  u[-2])_D1 + (f[-2, :, D1] - feq[-2, :, D1])
+      f[-1, :, D2] = E(rho_out, u[1])_D2  + (f[1,  :, D2] - feq[1,  :, D2])
+    The mask only picks the orientation: D1,D2 = (1,5,8),(3,6,7) when column
+    0 equals column -1, else (2,5,6),(4,7,8) (still written on x columns).
+    """
+    assert mask.dtype == bool
+    assert np.all(mask[0, :] == mask[-1, :]) or np.all(mask[:, 0] == mask[:, -1])
+    if np.all(mask[0, :] == mask[-1, :]):
+        n = mask.shape[1]
+        d1, d2 = (1, 5, 8), (3, 6, 7)
+    else:
+        n = mask.shape[0]
+        d1, d2 = (2, 5, 6), (4, 7, 8)
+    rho_in = np.full(n, np.divide(p_in, CS2))
+    rho_out = np.full(n, np.divide(p_out, CS2))
+
+    def bc(f_pre, rho, u):
+        assert mask.shape == f_pre.shape[0:2]
+        feq = equilibrium(rho, u)
+        e_in = equilibrium(rho_in, u[-2, ...])
+        for i in d1:
+            f_pre[0, :, i] = e_in[:, i] + (f_pre[-2, :, i] - feq[-2, :, i])
+        e_out = equilibrium(rho_out, u[1, ...])
+        for i in d2:
+            f_pre[-1, :, i] = e_out[:, i] + (f_pre[1, :, i] - feq[1, :, i])
+        return f_pre
+
+    bc.oracle_kind = ("pressure", (float(rho_in[0]), float(rho_out[0]), d1, d2))
+    return bc
+
+
+# --------------------------------------------------------------------------
+# composites (src/boundary_utils.py), single domain
+# --------------------------------------------------------------------------
+def _edge_mask(lx, ly, where):
+    m = np.zeros((lx, ly), dtype=bool)
+    if where == "top":
+        m[:, -1] = True
+    elif where == "bottom":
+        m[:, 0] = True
+    elif where == "left":
+        m[0, :] = True
+    elif where == "right":
+        m[-1, :] = True
+    return m
+
+
+def couette_flow_boundary_conditions(lx: int, ly: int, U: float, avg_density: float):
+    """Rigid top wall, then moving bottom wall u_w = (U, 0) (boundary_utils.py:9-55)."""
+    top = rigid_wall(_edge_mask(lx, ly, "top"))
+    bottom = moving_wall(_edge_mask(lx, ly, "bottom"), np.array([U, 0]), avg_density)
+
+    def boundary(f_pre, f_post, rho=None, u=None, f_prev=None):
+        f_post = top(f_pre, f_post)
+        f_post = bottom(f_pre, f_post)
+        return f_post
+
+    boundary.oracle_ops = [top.oracle_kind, bottom.oracle_kind]
+    return boundary
+
+
+def poiseuille_flow_boundary_conditions(lx: int, ly: int, p_in: float, p_out: float):
+    """Pressure fix on f_pre, re-stream, bottom wall, top wall (boundary_utils.py:58-110)."""
+    m = np.zeros((lx, ly), dtype=bool)
+    m[0, :] = True
+    m[-1, :] = True
+    pressure = periodic_with_pressure_variations(m, p_in, p_out)
+    bottom = rigid_wall(_edge_mask(lx, ly, "bottom"))
+    top = rigid_wall(_edge_mask(lx, ly, "top"))
+
+    def boundary(f_pre, f_post, rho, u, f_prev=None):
+        f_pre = pressure(f_pre, rho, u)
+        f_post = streaming(f_pre)
+        f_post = bottom(f_pre, f_post)
+        f_post = top(f_pre, f_post)
+        return f_post
+
+    boundary.oracle_ops = [pressure.oracle_kind, bottom.oracle_kind, top.oracle_kind]
+    return boundary
+
+
+def plate_mask(lx: int, ly: int, plate_size: int, y_offset: int = 0) -> np.ndarray:
+    """Plate column x = lx//4, rows [ly//2 - d//2, ly//2 + d//2) shifted by y_offset."""
+    m = np.zeros((lx, ly), dtype=bool)
+    y0 = ly // 2 - plate_size // 2 + y_offset
+    m[lx // 4, y0:y0 + 2 * (plate_size // 2)] = True
+    return m
+
+
+def von_karman_boundary_conditions(lx: int, ly: int, rho_in: float, u_in: float,
+                                   plate_size: int, walls: bool = False, y_offset: int = 0):
+    """Inlet, outlet, plate (boundary_utils.py:113-205 with one domain).
+
+    With walls=False and y_offset=0 this is the reference's periodic-in-y
+    channel (equal to milestoneQuickFunctionCalls.py:304-311).  walls=True adds
+    rigid bottom and top walls after the plate, and y_offset shifts the plate,
+    as in the 4200x1800 / 8192x2048 channel workloads.
+    """
+    bcs = [inlet((lx, ly), rho_in, u_in), outlet(),
+           rigid_object(plate_mask(lx, ly, plate_size, y_offset))]
+    if walls:
+        bcs.append(rigid_wall(_edge_mask(lx, ly, "bottom")))
+        bcs.append(rigid_wall(_edge_mask(lx, ly, "top")))
+
+    def boundary(f_pre, f_post, rho=None, u=None, f_prev=None):
+        f_post = bcs[0](f_post)
+        f_post = bcs[1](f_prev, f_post)
+        for b in bcs[2:]:
+            f_post = b(f_pre, f_post)
+        return f_post
+
+    boundary.oracle_ops = [b.oracle_kind for b in bcs]
+    return boundary
+
+
+# --------------------------------------------------------------------------
+# initial states (src/initial_values.py)
+# --------------------------------------------------------------------------
+def sinusoidal_density_x(shape, p0: float, eps: float):
+    """rho = p0 + eps*sin(2 pi x/lx), u = 0 (initial_values.py:38-64)."""
+    assert p0 + eps < 1
+    assert p0 - eps > 0
+    x = np.arange(shape[0])
+    col = p0 + eps * np.sin(np.divide(2 * np.pi * x, shape[0]))
+    rho = np.repeat(col[:, None], shape[1], axis=1)
+    return rho, np.zeros(tuple(shape) + (2,))
+
+
+def sinusoidal_velocity_x(shape, eps: float):
+    """rho = 1, ux = eps*sin(2 pi y/ly), uy = 0 (initial_values.py:67-93)."""
+    assert abs(eps) < 0.1
+    y = np.arange(shape[1])
+    row = eps * np.sin(np.divide(2 * np.pi * y, shape[1]))
+    u = np.zeros(tuple(shape) + (2,))
+    u[..., 0] = row[None, :]
+    return np.ones(tuple(shape)), u
+
+
+def density_1_velocity_0_initial(shape):
+    """rho = 1, u = 0 (initial_values.py:96-106)."""
+    return np.ones(tuple(shape)), np.zeros(tuple(shape) + (2,))
+
+
+def density_1_velocity_x_u0_velocity_y_0_initial(shape, u0: float):
+    """rho = 1, u = (u0, 0) (initial_values.py:109-123)."""
+    u = np.zeros(tuple(shape) + (2,))
+    u[..., 0] = u0
+    return np.ones(tuple(shape)), u
+
+
+# --------------------------------------------------------------------------
+# drivers and diagnostics
+# --------------------------------------------------------------------------
+def run(f, rho, u, omega, steps, boundary=None, probe=None):
+    """Run `steps` reference steps; optionally record |u| at probe=(x, y) after each."""
+    trace = []
+    for _ in range(steps):
+        f, rho, u = step(f, rho, u, omega, boundary)
+        if probe is not None:
+            trace.append(float(np.sqrt(u[probe][0] * u[probe][0] + u[probe][1] * u[probe][1])))
+    return f, rho, u, np.array(trace)
+
+
+def reynolds_number(L, u, nu):
+    """Re = L u / nu (lattice_boltzmann_method.py:55-71)."""
+    return np.divide(L * u, nu)
+
+
+def strouhal_number(freq, L, u):
+    """St = f L / u (lattice_boltzmann_method.py:74-90)."""
+    return np.divide(freq * L, u)
+
+
+def omega_from_viscosity(nu: float) -> float:
+    """omega = 1/(3 nu + 1/2) (experiments.py:672)."""
+    return float(np.reciprocal(3 * nu + 0.5))
+
+
+def fft_peak_frequency(trace: np.ndarray) -> float:
+    """|frequency| of the largest FFT magnitude after removing the mean
+    (visualizations_utils.py:157-160)."""
+    v = np.array(trace, dtype=np.float64)
+    v -= np.mean(v)
+    yf = np.fft.fft(v)
+    freq = np.fft.fftfreq(len(v), 1)
+    return float(np.abs(freq[np.argmax(np.abs(yf))]))
+
+
+def shear_wave_amplitude_decay(eps: float, nu: float, k: float, t) -> np.ndarray:
+    """Analytic amplitude eps*exp(-nu k^2 t) of a decaying shear wave."""
+    return eps * np.exp(-nu * k * k * np.asarray(t, dtype=np.float64))
+
+
+def viscosity_from_omega(omega: float) -> float:
+    """Lattice viscosity nu = (1/omega - 1/2)/3."""
+    return (1.0 / 3.0) * (1.0 / omega - 0.5)
