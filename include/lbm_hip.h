@@ -135,8 +135,10 @@ int lbm_read_probe(lbm_ctx *ctx, double *out, int64_t max, int64_t *count);
 int lbm_moments_sum(lbm_ctx *ctx, double out[3]);
 int lbm_synchronize(lbm_ctx *ctx);
 int lbm_steps_done(const lbm_ctx *ctx, int64_t *steps);
-/* Launch tuning of the fused step: cells per thread (1, 2 or 4), XCD-aware
- * block order (0/1), non-temporal stores (0/1).  Defaults are tuned values. */
+/* Launch tuning of the fused step: nodes per thread (1, 2 or 4; 0 = aligned
+ * 16-byte groups, needs ny divisible by 2/4/8 for FP64/FP32/FP16), XCD-aware
+ * block order (0/1), non-temporal stores (0/1).  Default {1, 1, 0}, the
+ * fastest measured on MI355X for every storage. */
 int lbm_set_launch(lbm_ctx *ctx, int cells_per_thread, int xcd_remap, int nt_stores);
 
 /* ---- stateless drop-ins on device arrays (AoS, C order) ----------------- */

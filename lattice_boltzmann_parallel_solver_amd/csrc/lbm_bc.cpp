@@ -260,7 +260,9 @@ extern "C" int lbm_bc_compile(lbm_bc_prog **out, int nx, int ny, const lbm_bc_de
     p->nx = nx;
     p->ny = ny;
     p->codes.assign(N, 0);
-    p->chunk_flags.assign((N + 63) / 64, 0);
+    // one flag per 64 nodes, padded to a multiple of 8 so a wave of the
+    // 16-byte kernel (up to 8 chunks) reads its flags with one aligned load
+    p->chunk_flags.assign(((N + 63) / 64 + 7) & ~size_t(7), 0);
     p->classes.push_back(ClassEntry{0, 0, {0}});
     p->rho_in = b.rho_in;
     p->rho_out = b.rho_out;

@@ -1,7 +1,10 @@
 // D2Q9 lattice-Boltzmann kernels for gfx950 (MI355X).
 //
-// State layout in HBM: nine y-contiguous population planes per buffer,
-// plane i at base + i*N, node n = x*ly + y (x slowest, as in the reference's
+// State layout in HBM: row-interleaved planes, population i of node (x, y) at
+// ((x*9 + i)*ly + y): every row stores its nine y-contiguous planes back to
+// back, so the three source rows of a wavefront are three contiguous blocks
+// (measured faster than nine whole-lattice planes, which stream from nine
+// regions at once).  Node n = x*ly + y (x slowest, as the reference's
 // f[x, y, i]); two buffers (A/B) alternate every step.  The hot kernel is a
 // pull-scheme fused step: each thread gathers the nine populations streaming
 // into its node, applies the node's boundary class, computes the moments,
@@ -100,20 +103,31 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
     return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
+// Offset of population i of node (x, y) in a state buffer.
+__device__ __forceinline__ size_t pop(const Geom &g, uint32_t x, int i, uint32_t y) {
+    return ((size_t)x * 9 + i) * g.ly + y;
+}
+
+// Source row of direction i for destination row x (periodic in x).
+__device__ __forceinline__ uint32_t src_row(const Geom &g, uint32_t x, int cx) {
+    if (cx == 1) return x == 0 ? g.lx - 1 : x - 1;
+    if (cx == -1) return x == g.lx - 1 ? 0 : x + 1;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t src_col(const Geom &g, uint32_t y, int cy) {
+    if (cy == 1) return y == 0 ? g.ly - 1 : y - 1;
+    if (cy == -1) return y == g.ly - 1 ? 0 : y + 1;
+    return y;
+}
+
 // Periodic pull: population i of node (x, y) comes from node (x - cx_i, y - cy_i).
 template <class S>
-__device__ __forceinline__ void pull(const S *__restrict__ src, const Geom &g, uint32_t n, uint32_t x,
-                                     uint32_t y, typename Io<S>::C out[9]) {
-    const bool x0 = x == 0, x1 = x == g.lx - 1, y0 = y == 0, y1 = y == g.ly - 1;
+__device__ __forceinline__ void pull(const S *__restrict__ src, const Geom &g, uint32_t x, uint32_t y,
+                                     typename Io<S>::C out[9]) {
 #pragma unroll
-    for (int i = 0; i < 9; i++) {
-        uint32_t m = n - (uint32_t)(kCX[i] * (int)g.ly + kCY[i]);
-        if (kCX[i] == 1 && x0) m += g.N;
-        if (kCX[i] == -1 && x1) m -= g.N;
-        if (kCY[i] == 1 && y0) m += g.ly;
-        if (kCY[i] == -1 && y1) m -= g.ly;
-        out[i] = Io<S>::ld(src + (size_t)i * g.N + m);
-    }
+    for (int i = 0; i < 9; i++)
+        out[i] = Io<S>::ld(src + pop(g, src_row(g, x, kCX[i]), i, src_col(g, y, kCY[i])));
 }
 
 // Apply the node's boundary class to the pulled populations; returns the
@@ -121,7 +135,7 @@ __device__ __forceinline__ void pull(const S *__restrict__ src, const Geom &g, u
 template <class S>
 __device__ __forceinline__ uint32_t apply_class(const BcView &bc, const S *__restrict__ src,
                                                 const typename Io<S>::C *__restrict__ side_prev,
-                                                const Geom &g, uint32_t n, uint32_t y,
+                                                const Geom &g, uint32_t n, uint32_t x, uint32_t y,
                                                 typename Io<S>::C v[9]) {
     using C = typename Io<S>::C;
     const uint32_t code = bc.codes[n];
@@ -132,7 +146,7 @@ __device__ __forceinline__ uint32_t apply_class(const BcView &bc, const S *__res
     for (int d = 0; d < 9; d++) {
         const uint32_t kind = (w >> (2 * d)) & 3u;
         if (kind == kSrcReflect) {
-            v[d] = Io<S>::ld(src + (size_t)kOpp[d] * g.N + n) - (C)ce.k[d];
+            v[d] = Io<S>::ld(src + pop(g, x, kOpp[d], y)) - (C)ce.k[d];
         } else if (kind == kSrcConst) {
             if constexpr (Io<S>::kDev) v[d] = (C)(ce.k[d] - kWd[d]);
             else v[d] = ce.k[d];
@@ -250,14 +264,14 @@ __device__ __forceinline__ void feq_rep(double rho_b, const NodeOut<S> &o, typen
 // writes E(rho_out, u) + (s - feq) into column lx-1 for directions D2.
 template <class S>
 __device__ __forceinline__ void store_node(S *__restrict__ dst, typename Io<S>::C *__restrict__ side_next,
-                                           const Geom &g, const BcView &bc, uint32_t n, uint32_t y,
+                                           const Geom &g, const BcView &bc, uint32_t x, uint32_t y,
                                            uint32_t flags, const typename Io<S>::C v[9],
                                            const NodeOut<S> &o, bool nt) {
     using C = typename Io<S>::C;
     const uint32_t skip = flags >> kSkipShift;
 #pragma unroll
     for (int i = 0; i < 9; i++)
-        if (!((skip >> i) & 1u)) Io<S>::st(dst + (size_t)i * g.N + n, o.s[i], nt);
+        if (!((skip >> i) & 1u)) Io<S>::st(dst + pop(g, x, i, y), o.s[i], nt);
     if (flags & kFlagOutletSide) {
         side_next[(size_t)y * 3 + 0] = v[3];
         side_next[(size_t)y * 3 + 1] = v[6];
@@ -270,14 +284,14 @@ __device__ __forceinline__ void store_node(S *__restrict__ dst, typename Io<S>::
             feq_rep<S>(bc.rho_in, o, e);
 #pragma unroll
             for (int i = 0; i < 9; i++)
-                if ((bc.d1 >> i) & 1u) Io<S>::st(dst + (size_t)i * g.N + y, e[i] + (o.s[i] - o.e[i]), false);
+                if ((bc.d1 >> i) & 1u) Io<S>::st(dst + pop(g, 0, i, y), e[i] + (o.s[i] - o.e[i]), false);
         }
         if (flags & kFlagPressureOut) {
             feq_rep<S>(bc.rho_out, o, e);
-            const uint32_t m = (g.lx - 1) * g.ly + y;
 #pragma unroll
             for (int i = 0; i < 9; i++)
-                if ((bc.d2 >> i) & 1u) Io<S>::st(dst + (size_t)i * g.N + m, e[i] + (o.s[i] - o.e[i]), false);
+                if ((bc.d2 >> i) & 1u)
+                    Io<S>::st(dst + pop(g, g.lx - 1, i, y), e[i] + (o.s[i] - o.e[i]), false);
         }
     }
 }
@@ -335,17 +349,17 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedParams<S> p) {
     const uint32_t blk = p.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const Geom &g = p.g;
     C v[V][9];
-    uint32_t node[V], ys[V];
+    uint32_t node[V], xs[V], ys[V];
     bool live[V];
 #pragma unroll
     for (int k = 0; k < V; k++) {
         node[k] = (blk * V + k) * kBlock + threadIdx.x;
         live[k] = node[k] < g.N;
-        ys[k] = 0;
+        xs[k] = ys[k] = 0;
         if (live[k]) {
-            const uint32_t x = row_of(g, node[k]);
-            ys[k] = node[k] - x * g.ly;
-            pull<S>(p.src, g, node[k], x, ys[k], v[k]);
+            xs[k] = row_of(g, node[k]);
+            ys[k] = node[k] - xs[k] * g.ly;
+            pull<S>(p.src, g, xs[k], ys[k], v[k]);
         }
     }
     double acc0 = 0, acc1 = 0, acc2 = 0;
@@ -355,11 +369,12 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedParams<S> p) {
         uint32_t flags = 0;
         if (p.bc.chunk) {
             const uint32_t chunk = __builtin_amdgcn_readfirstlane(node[k] >> 6);
-            if (p.bc.chunk[chunk]) flags = apply_class<S>(p.bc, p.src, p.side_prev, g, node[k], ys[k], v[k]);
+            if (p.bc.chunk[chunk])
+                flags = apply_class<S>(p.bc, p.src, p.side_prev, g, node[k], xs[k], ys[k], v[k]);
         }
         NodeOut<S> o;
         collide_node<S>(v[k], p.omega, o);
-        store_node<S>(p.dst, p.side_next, g, p.bc, node[k], ys[k], flags, v[k], o, p.nt != 0);
+        store_node<S>(p.dst, p.side_next, g, p.bc, xs[k], ys[k], flags, v[k], o, p.nt != 0);
         if constexpr (kProbe) {
             if ((int64_t)node[k] == p.probe_cell) {
                 p.probe_out[0] = o.rho;
@@ -371,6 +386,174 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedParams<S> p) {
             acc0 += o.rho;
             acc1 += o.jx;
             acc2 += o.jy;
+        }
+    }
+    if constexpr (kReduce) block_sum3(acc0, acc1, acc2, p.partials + 3 * (size_t)blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// the fused step with 16-byte accesses
+//
+// Each thread owns one aligned 16-byte group of every plane: V = 16/sizeof(S)
+// consecutive nodes along y (2 FP64, 4 FP32, 8 FP16), so every load and store
+// is an aligned dwordx4.  A plane shifted in y (cy = +-1) needs one element
+// outside the group; it comes from the neighbouring lane's group by a lane
+// shuffle, or, for the first/last lane of a wave and at the row ends, from a
+// predicated single-element load.  Requires ly % V == 0.
+// ---------------------------------------------------------------------------
+using raw4 = unsigned int __attribute__((ext_vector_type(4)));
+
+template <class S>
+struct Pack;
+
+template <>
+struct Pack<double> {
+    static constexpr int V = 2;
+    __device__ static double get(const raw4 &r, int k) {
+        return __hiloint2double((int)r[2 * k + 1], (int)r[2 * k]);
+    }
+    __device__ static void set(raw4 &r, int k, double v) {
+        r[2 * k] = (unsigned)__double2loint(v);
+        r[2 * k + 1] = (unsigned)__double2hiint(v);
+    }
+    __device__ static double from_prev(const raw4 &r) {  // element V-1 of lane-1
+        const unsigned lo = __shfl_up(r[2], 1, 64), hi = __shfl_up(r[3], 1, 64);
+        return __hiloint2double((int)hi, (int)lo);
+    }
+    __device__ static double from_next(const raw4 &r) {  // element 0 of lane+1
+        const unsigned lo = __shfl_down(r[0], 1, 64), hi = __shfl_down(r[1], 1, 64);
+        return __hiloint2double((int)hi, (int)lo);
+    }
+};
+
+template <>
+struct Pack<float> {
+    static constexpr int V = 4;
+    __device__ static float get(const raw4 &r, int k) { return __uint_as_float(r[k]); }
+    __device__ static void set(raw4 &r, int k, float v) { r[k] = __float_as_uint(v); }
+    __device__ static float from_prev(const raw4 &r) { return __uint_as_float(__shfl_up(r[3], 1, 64)); }
+    __device__ static float from_next(const raw4 &r) { return __uint_as_float(__shfl_down(r[0], 1, 64)); }
+};
+
+template <>
+struct Pack<__half> {
+    static constexpr int V = 8;
+    __device__ static float half_bits(unsigned b) { return __half2float(__ushort_as_half((unsigned short)b)); }
+    __device__ static float get(const raw4 &r, int k) { return half_bits((r[k >> 1] >> (16 * (k & 1))) & 0xffffu); }
+    __device__ static void set(raw4 &r, int k, float v) {
+        const unsigned b = __half_as_ushort(__float2half_rn(v));
+        const int sh = 16 * (k & 1);
+        r[k >> 1] = (r[k >> 1] & ~(0xffffu << sh)) | (b << sh);
+    }
+    __device__ static float from_prev(const raw4 &r) { return half_bits(__shfl_up(r[3], 1, 64) >> 16); }
+    __device__ static float from_next(const raw4 &r) { return half_bits(__shfl_down(r[0], 1, 64) & 0xffffu); }
+};
+
+template <class S, bool kProbe, bool kReduce>
+__global__ __launch_bounds__(kBlock) void k_fused16(FusedParams<S> p) {
+    using C = typename Io<S>::C;
+    using P = Pack<S>;
+    constexpr int V = P::V;
+    const Geom &g = p.g;
+    const uint32_t blk = p.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t t = blk * kBlock + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    const bool live = (uint64_t)t * V < g.N;
+    const uint32_t n0 = live ? t * V : 0;  // dead lanes load node 0 and store nothing
+    const uint32_t x = row_of(g, n0), y0 = n0 - x * g.ly;
+
+    // 1. one aligned 16-byte load per plane
+    raw4 r[9];
+    const S *rows[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        rows[i] = p.src + pop(g, src_row(g, x, kCX[i]), i, 0);
+        r[i] = *reinterpret_cast<const raw4 *>(rows[i] + y0);
+    }
+    // 2. the element each y-shifted plane needs from outside the group
+    C edge[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        if (kCY[i] == 1) edge[i] = P::from_prev(r[i]);
+        else if (kCY[i] == -1) edge[i] = P::from_next(r[i]);
+    }
+    const bool fix_prev = lane == 0 || y0 == 0;
+    const bool fix_next = lane == 63 || y0 + V == g.ly;
+    if (fix_prev || fix_next) {
+        const uint32_t yp = y0 == 0 ? g.ly - 1 : y0 - 1;
+        const uint32_t yn = y0 + V == g.ly ? 0 : y0 + V;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            if (kCY[i] == 1 && fix_prev) edge[i] = Io<S>::ld(rows[i] + yp);
+            if (kCY[i] == -1 && fix_next) edge[i] = Io<S>::ld(rows[i] + yn);
+        }
+    }
+    // 3. boundary classes are looked up only in waves that hold a special node
+    bool special = false;
+    if (p.bc.chunk) {
+        const uint32_t wave_first = (t - lane) * V;
+        if (wave_first < g.N) {
+            const uint32_t c0 = __builtin_amdgcn_readfirstlane(wave_first >> 6);
+            uint32_t any = 0;
+#pragma unroll
+            for (int j = 0; j < V; j++) any |= p.bc.chunk[c0 + j];
+            special = any != 0;
+        }
+    }
+    // 4. collide node by node
+    raw4 o[9];
+    uint32_t skip_any = 0, skip_node[V];
+    double acc0 = 0, acc1 = 0, acc2 = 0;
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+        C v[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            if (kCY[i] == 0) v[i] = P::get(r[i], k);
+            else if (kCY[i] == 1) v[i] = k == 0 ? edge[i] : P::get(r[i], k - 1);
+            else v[i] = k == V - 1 ? edge[i] : P::get(r[i], k + 1);
+        }
+        const uint32_t n = n0 + k, y = y0 + k;
+        uint32_t flags = 0;
+        if (special && live) flags = apply_class<S>(p.bc, p.src, p.side_prev, g, n, x, y, v);
+        NodeOut<S> out;
+        collide_node<S>(v, p.omega, out);
+#pragma unroll
+        for (int i = 0; i < 9; i++) P::set(o[i], k, out.s[i]);
+        skip_node[k] = flags >> kSkipShift;
+        skip_any |= skip_node[k];
+        if (flags & ~(0x1ffu << kSkipShift)) {
+            // outlet buffer and pressure fix (store_node without the plane stores)
+            store_node<S>(p.dst, p.side_next, g, p.bc, x, y, flags | (0x1ffu << kSkipShift), v, out, false);
+        }
+        if constexpr (kProbe) {
+            if (live && (int64_t)n == p.probe_cell) {
+                p.probe_out[0] = out.rho;
+                p.probe_out[1] = (double)out.ux;
+                p.probe_out[2] = (double)out.uy;
+            }
+        }
+        if constexpr (kReduce) {
+            if (live) {
+                acc0 += out.rho;
+                acc1 += out.jx;
+                acc2 += out.jy;
+            }
+        }
+    }
+    // 5. one aligned 16-byte store per plane (per node where a pressure fix owns a direction)
+    if (live) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            S *dst = p.dst + pop(g, x, i, y0);
+            if (!((skip_any >> i) & 1u)) {
+                if (p.nt) __builtin_nontemporal_store(o[i], reinterpret_cast<raw4 *>(dst));
+                else *reinterpret_cast<raw4 *>(dst) = o[i];
+            } else {
+#pragma unroll
+                for (int k = 0; k < V; k++)
+                    if (!((skip_node[k] >> i) & 1u)) Io<S>::st(dst + k, P::get(o[i], k), false);
+            }
         }
     }
     if constexpr (kReduce) block_sum3(acc0, acc1, acc2, p.partials + 3 * (size_t)blockIdx.x);
@@ -400,7 +583,7 @@ __global__ __launch_bounds__(kBlock) void k_collide_first(const double *__restri
     const uint32_t skip = flags >> kSkipShift;
 #pragma unroll
     for (int i = 0; i < 9; i++)
-        if (!((skip >> i) & 1u)) Io<S>::st(dst + (size_t)i * g.N + n, rep<S>(s[i], i), false);
+        if (!((skip >> i) & 1u)) Io<S>::st(dst + pop(g, x, i, y), rep<S>(s[i], i), false);
     if (flags & kFlagOutletSide) {
         side[(size_t)y * 3 + 0] = rep<S>(fi[3], 3);
         side[(size_t)y * 3 + 1] = rep<S>(fi[6], 6);
@@ -410,14 +593,14 @@ __global__ __launch_bounds__(kBlock) void k_collide_first(const double *__restri
         double ein[9];
         feq_f64(bc.rho_in, ux, uy, ein);
         for (int i = 0; i < 9; i++)
-            if ((bc.d1 >> i) & 1u) Io<S>::st(dst + (size_t)i * g.N + y, rep<S>(ein[i] + (s[i] - e[i]), i), false);
+            if ((bc.d1 >> i) & 1u) Io<S>::st(dst + pop(g, 0, i, y), rep<S>(ein[i] + (s[i] - e[i]), i), false);
     }
     if (flags & kFlagPressureOut) {
         double eout[9];
         feq_f64(bc.rho_out, ux, uy, eout);
-        const uint32_t m = (g.lx - 1) * g.ly + y;
         for (int i = 0; i < 9; i++)
-            if ((bc.d2 >> i) & 1u) Io<S>::st(dst + (size_t)i * g.N + m, rep<S>(eout[i] + (s[i] - e[i]), i), false);
+            if ((bc.d2 >> i) & 1u)
+                Io<S>::st(dst + pop(g, g.lx - 1, i, y), rep<S>(eout[i] + (s[i] - e[i]), i), false);
     }
 }
 
@@ -431,8 +614,8 @@ __device__ __forceinline__ void finalize_node(const S *__restrict__ src, const t
     using C = typename Io<S>::C;
     const uint32_t x = row_of(g, n), y = n - x * g.ly;
     C v[9];
-    pull<S>(src, g, n, x, y, v);
-    if (bc.codes) apply_class<S>(bc, src, side_prev, g, n, y, v);
+    pull<S>(src, g, x, y, v);
+    if (bc.codes) apply_class<S>(bc, src, side_prev, g, n, x, y, v);
 #pragma unroll
     for (int i = 0; i < 9; i++) {
         if constexpr (Io<S>::kDev) f[i] = (double)v[i] + kWd[i];
@@ -474,8 +657,8 @@ __global__ void k_probe_cell(const S *__restrict__ src, const typename Io<S>::C 
         using C = typename Io<S>::C;
         const uint32_t x = row_of(g, n), y = n - x * g.ly;
         C v[9];
-        pull<S>(src, g, n, x, y, v);
-        if (bc.codes) apply_class<S>(bc, src, side_prev, g, n, y, v);
+        pull<S>(src, g, x, y, v);
+        if (bc.codes) apply_class<S>(bc, src, side_prev, g, n, x, y, v);
         NodeOut<S> o;
         collide_node<S>(v, 1.0, o);
         out[0] = o.rho;
@@ -647,6 +830,18 @@ hipError_t fused_typed(const Geom &g, const StateView &st, int src, const BcView
     p.xcd = cfg.xcd_remap;
     p.nt = cfg.nt_stores;
     const int V = cfg.cells_per_thread;
+    if (V == 0) {
+        constexpr int V16 = Pack<S>::V;
+        if (g.ly % V16 != 0) return hipErrorInvalidValue;
+        const unsigned grid = (unsigned)(((int64_t)g.N / V16 + kBlock - 1) / kBlock);
+        if (nblocks) *nblocks = (int)grid;
+        const bool reduce = partials != nullptr;
+        if (probe && reduce) hipLaunchKernelGGL((k_fused16<S, true, true>), dim3(grid), dim3(kBlock), 0, s, p);
+        else if (probe) hipLaunchKernelGGL((k_fused16<S, true, false>), dim3(grid), dim3(kBlock), 0, s, p);
+        else if (reduce) hipLaunchKernelGGL((k_fused16<S, false, true>), dim3(grid), dim3(kBlock), 0, s, p);
+        else hipLaunchKernelGGL((k_fused16<S, false, false>), dim3(grid), dim3(kBlock), 0, s, p);
+        return hipGetLastError();
+    }
     const unsigned grid = (unsigned)(((int64_t)g.N + (int64_t)kBlock * V - 1) / ((int64_t)kBlock * V));
     if (nblocks) *nblocks = (int)grid;
     switch (V) {
@@ -659,8 +854,10 @@ hipError_t fused_typed(const Geom &g, const StateView &st, int src, const BcView
 
 }  // namespace
 
+int vec16_nodes(int storage) { return storage == LBM_STORE_F64 ? 2 : (storage == LBM_STORE_F32 ? 4 : 8); }
+
 int fused_blocks(const Geom &g, const LaunchCfg &cfg) {
-    const int64_t per = (int64_t)kBlock * cfg.cells_per_thread;
+    const int64_t per = (int64_t)kBlock * (cfg.cells_per_thread ? cfg.cells_per_thread : 1);
     return (int)(((int64_t)g.N + per - 1) / per);
 }
 

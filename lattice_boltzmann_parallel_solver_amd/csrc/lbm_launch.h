@@ -42,7 +42,7 @@ struct ProbeView {
 };
 
 struct LaunchCfg {
-    int cells_per_thread = 2;
+    int cells_per_thread = 0;  // 0: 16-byte kernel (needs ly % (16/sizeof) == 0), else 1, 2, 4
     int xcd_remap = 1;
     int nt_stores = 0;
 };
@@ -53,6 +53,8 @@ hipError_t launch_fused(int storage, const Geom &g, const StateView &st, int src
                         double omega, const ProbeView &pr, int64_t probe_slot, double *partials,
                         const LaunchCfg &cfg, hipStream_t s, int *nblocks_out);
 int fused_blocks(const Geom &g, const LaunchCfg &cfg);
+// Nodes per thread of the 16-byte kernel for a storage type.
+int vec16_nodes(int storage);
 
 // First collision from AoS FP64 (f, rho, u) into planes[dst] / side[dst].
 hipError_t launch_collide_first(int storage, const Geom &g, const double *f, const double *rho,

@@ -150,7 +150,9 @@ class Lattice:
         check(_lib.load().lbm_set_bc(self._h, prog.handle if prog is not None else None))
         self._program = prog
 
-    def set_launch(self, cells_per_thread: int = 2, xcd_remap: bool = True, nt_stores: bool = False) -> None:
+    def set_launch(self, cells_per_thread: int = 1, xcd_remap: bool = True, nt_stores: bool = False) -> None:
+        """Launch shape of the fused step: nodes per thread (1, 2, 4; 0 = aligned
+        16-byte groups), XCD-aware block order, non-temporal stores."""
         check(_lib.load().lbm_set_launch(self._h, int(cells_per_thread), int(bool(xcd_remap)), int(bool(nt_stores))))
 
     # -- state ------------------------------------------------------------------

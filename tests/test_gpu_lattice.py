@@ -56,13 +56,14 @@ def test_dropin_step_bit_exact(name):
         assert np.array_equal(fg, fo) and np.array_equal(rg, ro) and np.array_equal(ug, uo)
 
 
-@pytest.mark.parametrize("storage_cfg", [(1, 0, 0), (2, 1, 0), (4, 1, 1), (2, 0, 1)])
+@pytest.mark.parametrize("storage_cfg", [(0, 0, 0), (0, 1, 1), (1, 0, 0), (2, 1, 0), (4, 1, 1), (2, 0, 1)])
 @pytest.mark.parametrize("name", NAMES)
 def test_lattice_run_bit_exact(name, storage_cfg):
-    """n fused steps on the device equal n oracle steps, for every launch shape."""
+    """n fused steps on the device equal n oracle steps, for every launch shape
+    (0 = 16-byte groups of 2 FP64 nodes, else nodes per thread)."""
     from lattice_boltzmann_parallel_solver_amd import Lattice
 
-    lx, ly = 130, 71
+    lx, ly = 130, 72
     _, bo, mk, f, rho, u, om = pick(name, lx, ly)
     lat = Lattice(lx, ly, boundary=mk(), storage="f64")
     v, xcd, nt = storage_cfg
@@ -71,6 +72,23 @@ def test_lattice_run_bit_exact(name, storage_cfg):
     lat.run(om, 150)
     got = lat.state()
     want = N.run(f, rho, u, om, 150, bo)
+    for a, b in zip(got, want[:3]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("shape", [(57, 33), (16, 2), (9, 1), (3, 6)])
+def test_ragged_and_thin_lattices_bit_exact(shape):
+    """Rows that do not split into 16-byte groups use the per-node kernel."""
+    from lattice_boltzmann_parallel_solver_amd import Lattice
+
+    lx, ly = shape
+    rho, u = O.sinusoidal_velocity_x((lx, ly), 0.05)
+    f = O.equilibrium(rho, u)
+    lat = Lattice(lx, ly)
+    lat.set_state(f, rho, u)
+    lat.run(0.9, 25)
+    got = lat.state()
+    want = N.run(f, rho, u, 0.9, 25)
     for a, b in zip(got, want[:3]):
         assert np.array_equal(a, b)
 
