@@ -7,8 +7,11 @@ top and bottom walls, nu = 0.1*400/100 (Re = 100) so omega = 1/1.7.
 
 value      : MLUPS of the FP32-deviation step (inputs resident in HBM).
 roofline   : HBM bound; achieved = algorithmic bytes per launch (18 population
-             accesses x 4 B per node) / average launch time over the timed region.
-fp64       : the same workload with FP64 populations.
+             accesses x 4 B per node) / average launch time over the timed region
+             (HIP events on the stream the kernels run on); traffic = HBM bytes per
+             launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated
+             (profiles/fused_step_8192x2048_hbm_traffic.json).
+fp64, f16  : the same workload with FP64 populations / FP16 deviation storage.
 cpu_baseline: the oracle (numpy FP64, one core) on a few steps of the same workload.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
@@ -29,6 +32,16 @@ HBM_PEAK_GBS = 8000.0
 LX, LY, PLATE = 8192, 2048, 400
 U_IN, RE = 0.1, 100.0
 BYTES_PER_NODE = {"f64": 144, "f32": 72, "f16": 36}
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fused_step_8192x2048_hbm_traffic.json")
+
+
+def measured_traffic(storage):
+    """HBM bytes per fused-step launch from the committed rocprofv3 counter runs."""
+    try:
+        with open(TRAFFIC_JSON) as fh:
+            return float(json.load(fh)[storage]["traffic_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def omega():
@@ -89,7 +102,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "f64"])
     ap.add_argument("--fp64-steps", type=int, default=200)
+    ap.add_argument("--f16-steps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 3 device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -101,6 +117,14 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        a = torch.empty(args.copy_calibration // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": a.numel() * 4, "copies": 3}))
+        return
     N = LX * LY
 
     lat = build_lattice(args.storage, torch)
@@ -116,18 +140,20 @@ def main():
     bytes_launch = BYTES_PER_NODE[args.storage] * N
     achieved = bytes_launch / launch_s / 1e9
 
-    fp64 = None
-    if args.fp64_steps > 0 and args.storage != "f64":
-        lat = build_lattice("f64", torch)
-        ms64 = time_steps(lat, args.fp64_steps, max(5, args.warmup // 4), torch)
+    def leg(storage, steps):
+        if steps <= 0 or storage == args.storage:
+            return None
+        lat = build_lattice(storage, torch)
+        ms_ = time_steps(lat, steps, max(5, args.warmup // 4), torch)
         lat.close()
-        del lat
         torch.cuda.empty_cache()
-        s64 = ms64 / 1e3
-        ach64 = BYTES_PER_NODE["f64"] * N / (s64 / args.fp64_steps) / 1e9
-        fp64 = {"mlups": N * args.fp64_steps / s64 / 1e6, "ms_per_step": ms64 / args.fp64_steps,
-                "steps": args.fp64_steps, "roofline_achieved_gbs": ach64,
-                "roofline_frac": ach64 / HBM_PEAK_GBS, "bytes_per_node": BYTES_PER_NODE["f64"]}
+        ach = BYTES_PER_NODE[storage] * N / (ms_ / 1e3 / steps) / 1e9
+        return {"mlups": N * steps / (ms_ / 1e3) / 1e6, "ms_per_step": ms_ / steps, "steps": steps,
+                "roofline_achieved_gbs": ach, "roofline_frac": ach / HBM_PEAK_GBS,
+                "bytes_per_node": BYTES_PER_NODE[storage], "traffic": measured_traffic(storage)}
+
+    fp64 = leg("f64", args.fp64_steps)
+    f16 = leg("f16", args.f16_steps)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline()
 
@@ -149,9 +175,10 @@ def main():
                    "lx": LX, "ly": LY, "storage": args.storage, "omega": omega(),
                    "parallelism": "single GPU (whole lattice resident)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.storage),
                      "bytes_per_node": BYTES_PER_NODE[args.storage]},
         "fp64": fp64,
+        "f16": f16,
         "latency": None,
         "cpu_baseline": cpu,
     }
