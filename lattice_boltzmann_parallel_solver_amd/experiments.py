@@ -1,0 +1,208 @@
+"""Experiment drivers of the reference, on the device-resident lattice.
+
+Counterparts of src/experiments.py and src/visualizations_utils.py without
+the plotting: each function runs the reference's scenario with the fused GPU
+step and returns the numbers the reference plotted or wrote to CSV.
+
+  shear_wave_viscosity     plot_measured_viscosity_vs_omega, velocity variant
+                           (experiments.py:147-223)
+  couette_profile          plot_couette_flow_vel_vectors (experiments.py:304-374)
+  poiseuille_profile       plot_poiseuille_flow_vel_vectors (experiments.py:377-504)
+  von_karman_probe         x_strouhal (experiments.py:650-720), optionally with the
+                           rigid walls and plate offset of the large channel runs
+  strouhal_reference       the FFT peak procedure (visualizations_utils.py:149-167)
+  strouhal_interpolated,
+  strouhal_zero_crossing   sub-bin estimators (the reference's peak is quantised
+                           to one FFT bin, d/(u N) in Strouhal units)
+"""
+
+from __future__ import annotations
+
+from typing import Optional, Tuple
+
+import numpy as np
+
+from .boundary_utils import (couette_flow_boundary_conditions, poiseuille_flow_boundary_conditions,
+                             von_karman_boundary_conditions)
+from .initial_values import (density_1_velocity_0_initial, density_1_velocity_x_u0_velocity_y_0_initial,
+                             sinusoidal_velocity_x)
+from .solver import Lattice
+
+
+def omega_from_viscosity(nu: float) -> float:
+    """omega = 1/(3 nu + 1/2) (experiments.py:672)."""
+    return float(np.reciprocal(3 * nu + 0.5))
+
+
+def viscosity_from_omega(omega: float) -> float:
+    """nu = (1/omega - 1/2)/3 (experiments.py:210)."""
+    return (1 / 3) * (1 / omega - 0.5)
+
+
+def speed(samples: np.ndarray) -> np.ndarray:
+    """|u| of probe rows (rho, ux, uy) as sqrt(ux^2 + uy^2)."""
+    ux, uy = samples[:, 1], samples[:, 2]
+    return np.sqrt(ux * ux + uy * uy)
+
+
+# ----------------------------------------------------------------------------- Strouhal
+def strouhal_reference(trace, L: float, u: float, start: int = 0) -> Tuple[float, float, float]:
+    """(St, peak frequency, bin width in St) by the reference's procedure:
+    remove the mean of trace[start:], take |FFT| and its argmax
+    (visualizations_utils.py:157-167)."""
+    v = np.array(trace[start:], dtype=np.float64)
+    v -= np.mean(v)
+    yf = np.fft.fft(v)
+    freq = np.fft.fftfreq(len(v), 1)
+    f = float(np.abs(freq[np.argmax(np.abs(yf))]))
+    return f * L / u, f, L / (u * len(v))
+
+
+def strouhal_interpolated(trace, L: float, u: float, start: int = 0) -> float:
+    """St from a Hann-windowed spectrum with the peak located by a parabola
+    through the log magnitudes of the three bins around it (quadratic
+    interpolation of the Gaussian-like Hann main lobe)."""
+    v = np.array(trace[start:], dtype=np.float64)
+    v -= np.mean(v)
+    n = len(v)
+    mag = np.abs(np.fft.rfft(v * np.hanning(n)))
+    k = int(np.argmax(mag[1:])) + 1
+    if 1 <= k < len(mag) - 1:
+        a, b, c = np.log(mag[k - 1] + 1e-300), np.log(mag[k] + 1e-300), np.log(mag[k + 1] + 1e-300)
+        den = a - 2 * b + c
+        delta = 0.5 * (a - c) / den if den != 0 else 0.0
+    else:
+        delta = 0.0
+    return (k + delta) / n * L / u
+
+
+def strouhal_zero_crossing(trace, L: float, u: float, start: int = 0) -> float:
+    """St from the mean period between upward crossings of the mean (crossing
+    times linearly interpolated)."""
+    v = np.array(trace[start:], dtype=np.float64)
+    v -= np.mean(v)
+    idx = np.flatnonzero((v[:-1] < 0) & (v[1:] >= 0))
+    if len(idx) < 2:
+        return float("nan")
+    t = idx + v[idx] / (v[idx] - v[idx + 1])
+    period = (t[-1] - t[0]) / (len(t) - 1)
+    return L / (period * u)
+
+
+# ----------------------------------------------------------------------------- drivers
+def von_karman_probe(lx: int, ly: int, plate_size: int, steps: int, nu: float = 0.04, u_in: float = 0.1,
+                     rho_in: float = 1.0, storage: str = "f64", walls: bool = False, y_offset: int = 0,
+                     probe: Optional[Tuple[int, int]] = None, device: int = 0, chunk: int = 100000):
+    """x_strouhal (experiments.py:650-720) on one GPU: uniform inflow start,
+    probe (rho, ux, uy) at (3 lx//4, ly//2) after every step.
+
+    Returns (samples[steps, 3], lattice)."""
+    probe = probe or (3 * lx // 4, ly // 2)
+    lat = Lattice(lx, ly, boundary=von_karman_boundary_conditions(lx, ly, rho_in, u_in, plate_size, walls=walls,
+                                                                  y_offset=y_offset),
+                  storage=storage, device=device)
+    rho, u = density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), u_in)
+    lat.set_equilibrium(rho, u)
+    lat.set_probe(probe[0], probe[1], steps)
+    om = omega_from_viscosity(nu)
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        lat.run(om, k)
+        done += k
+    lat.synchronize()
+    return lat.probe(), lat
+
+
+def shear_wave_viscosity(lx: int, ly: int, omega: float, epsilon: float = 0.05, steps: int = 3000,
+                         sample_every: int = 50, storage: str = "f64", device: int = 0):
+    """Decay of ux = eps sin(2 pi y / ly) (experiments.py:147-223, velocity
+    variant): amplitude max |u| sampled every `sample_every` steps, fitted to
+    eps exp(-nu k^2 t).  Returns (nu_fit, nu_theory, t, amplitude)."""
+    from scipy.optimize import curve_fit
+
+    lat = Lattice(lx, ly, storage=storage, device=device)
+    rho, u = sinusoidal_velocity_x((lx, ly), epsilon)
+    lat.set_equilibrium(rho, u)
+    t, amp = [], []
+    done = 0
+    while done < steps:
+        k = min(sample_every, steps - done)
+        lat.run(omega, k)
+        done += k
+        _, _, uu = lat.state(want=("u",))
+        t.append(done)
+        amp.append(max(abs(uu.min()), abs(uu.max())))
+    t = np.array(t, dtype=np.float64)
+    amp = np.array(amp)
+    kk = 2 * np.pi / ly
+    popt, _ = curve_fit(lambda tt, v: epsilon * np.exp(-v * kk * kk * tt), t, amp, p0=[viscosity_from_omega(omega)])
+    return float(popt[0]), viscosity_from_omega(omega), t, amp
+
+
+def couette_analytic(ly: int, U: float, nu: float, t: float, terms: int = 4000) -> np.ndarray:
+    """Start-up Couette flow u_x(y, t) between a wall moving with U at
+    y = -1/2 and a rigid wall at y = ly - 1/2 (the bounce-back walls sit half a
+    node outside the boundary nodes), from rest: steady line minus the
+    Fourier series of the initial deficit."""
+    h = float(ly)
+    eta = np.arange(ly) + 0.5  # distance from the moving wall
+    u = U * (1 - eta / h)
+    for n in range(1, terms + 1):
+        u -= (2 * U / (n * np.pi)) * np.sin(n * np.pi * eta / h) * np.exp(-nu * (n * np.pi / h) ** 2 * t)
+    return u
+
+
+def couette_profile(lx: int, ly: int, U: float = 0.05, omega: float = 1.0, steps: int = 5000,
+                    storage: str = "f64", device: int = 0):
+    """plot_couette_flow_vel_vectors (experiments.py:304-363): u_x(y) at
+    x = lx/2 after `steps` steps, its linear regression, and the analytic
+    start-up profile.  Returns dict."""
+    from scipy.stats import linregress
+
+    lat = Lattice(lx, ly, boundary=couette_flow_boundary_conditions(lx, ly, U, 1.0), storage=storage,
+                  device=device)
+    rho, u = density_1_velocity_0_initial((lx, ly))
+    lat.set_equilibrium(rho, u)
+    lat.run(omega, steps)
+    _, rho, u = lat.state(want=("rho", "u"))
+    ux = u[int(lx / 2), :, 0]
+    r = linregress(np.arange(ly), ux)
+    ana = couette_analytic(ly, U, viscosity_from_omega(omega), steps)
+    return {"ux": ux, "slope": r.slope, "intercept": r.intercept, "rvalue": r.rvalue, "analytic": ana,
+            "max_abs_error": float(np.abs(ux - ana).max()), "rho": rho}
+
+
+def poiseuille_analytic(ly: int, dpdx: float, mu: float) -> np.ndarray:
+    """Steady Poiseuille profile -dp/dx/(2 mu) y (h - y) at y + 1/2, walls half a
+    node outside the boundary nodes (experiments.py:430-437)."""
+    y = np.arange(ly) + 0.5
+    return -np.reciprocal(2 * mu) * dpdx * y * (ly - y)
+
+
+def poiseuille_profile(lx: int, ly: int, delta_p: float = 0.001, omega: float = 1.5, steps: int = 40000,
+                       storage: str = "f64", device: int = 0):
+    """plot_poiseuille_flow_vel_vectors (experiments.py:377-504): pressure
+    difference split +-delta_p/2 around rho = 1; returns the profiles at
+    x = 1 and x = lx//2, the flux ratio, a quadratic fit and the analytic
+    profile."""
+    from scipy.optimize import curve_fit
+
+    drho = delta_p * 3
+    p_in, p_out = (1 + drho / 2) / 3, (1 - drho / 2) / 3
+    lat = Lattice(lx, ly, boundary=poiseuille_flow_boundary_conditions(lx, ly, p_in, p_out), storage=storage,
+                  device=device)
+    rho, u = density_1_velocity_0_initial((lx, ly))
+    lat.set_equilibrium(rho, u)
+    lat.run(omega, steps)
+    _, rho, u = lat.state(want=("rho", "u"))
+    vx = u[..., 0]
+    trapz = getattr(np, "trapezoid", None) or np.trapz
+    a_in = trapz(vx[1, :], np.arange(ly))
+    a_mid = trapz(vx[lx // 2, :], np.arange(ly))
+    popt, _ = curve_fit(lambda y, a, b, c: a * y ** 2 + b * y + c, np.arange(ly), vx[lx // 2, :])
+    mu = viscosity_from_omega(omega) * np.mean(rho[lx // 2, :])
+    ana = poiseuille_analytic(ly, (p_out - p_in) / lx, mu)
+    return {"ux_inlet": vx[1, :], "ux_mid": vx[lx // 2, :], "area_inlet": a_in, "area_middle": a_mid,
+            "ratio": a_in / a_mid, "popt": popt, "analytic": ana,
+            "max_rel_error": float(np.abs(vx[lx // 2, :] - ana).max() / ana.max()), "rho": rho}

@@ -14,6 +14,13 @@ GPU machines do not have); the committed outputs are data:
         figures/von_karman_vortex_shedding/velocity.py.npy   final velocity
       MPI run, 200k steps (experiments.py:650-720):
         figures/von_karman_vortex_shedding/reynold_strouhal/vel_at_p_100.npy
+  von_karman_sweeps.json             every probe trace of the reference's Strouhal
+                                     sweeps (figures/von_karman_vortex_shedding/
+                                     {reynold,nx,blockage}_strouhal/vel_at_p_*.npy,
+                                     main.py:150-207): run parameters, length,
+                                     SHA-256, every 1000th sample and the Strouhal
+                                     number by the reference's FFT procedure
+                                     (visualizations_utils.py:135-250).
   reference_csv.json                 the Couette / Poiseuille fits the reference
                                      published (figures/couette_flow/linregress.csv,
                                      figures/poiseuille_flow/{areas,curve_fit}.csv).
@@ -22,6 +29,7 @@ Digests are SHA-256 of the arrays' little-endian float64 bytes in C order.
 """
 
 import csv
+import glob
 import hashlib
 import json
 import os
@@ -64,6 +72,37 @@ def main():
         parallel_trace_head=par[:6001],
         parallel_trace_every100=par[::100],
     )
+
+    sweeps = []
+    for folder in ("reynold_strouhal", "nx_strouhal", "blockage_strouhal"):
+        for path in sorted(glob.glob(os.path.join(VK, folder, "vel_at_p_*.npy"))):
+            tag = path[path.rfind("_") + 1:path.rfind(".npy")]
+            tr = np.load(path)
+            lx, ly, nu, u0 = 420, 180, 0.04, 0.1
+            if folder == "reynold_strouhal":
+                re = int(tag)
+                if re > 130:   # main.py:154-165
+                    nu = 0.03
+                    u0 = re * nu / 40
+                else:
+                    nu = 40 * 0.1 / re
+                window = 150000 if re == 40 else (125000 if re == 70 else 70000)  # visualizations_utils.py:150-155
+            elif folder == "nx_strouhal":
+                lx = int(tag)
+                window = {260: 125000, 300: 90000, 350: 90000, 700: 90000, 100: 175000}.get(lx, 75000)
+            else:
+                ly = int(round(40 / float(tag)))
+                window = 90000
+            v = tr[window:].copy()
+            v -= np.mean(v)
+            freq = np.fft.fftfreq(len(v), 1)
+            f_peak = float(np.abs(freq[np.argmax(np.abs(np.fft.fft(v)))]))
+            sweeps.append({"folder": folder, "file": os.path.basename(path), "lx": lx, "ly": ly, "plate": 40,
+                           "nu": nu, "u0": u0, "steps": len(tr) - 1, "probe": [3 * lx // 4, ly // 2],
+                           "sha256": sha(tr), "every1000": tr[::1000].tolist(), "window": window,
+                           "peak_frequency": f_peak, "strouhal": f_peak * 40 / u0})
+    with open(os.path.join(OUT, "von_karman_sweeps.json"), "w") as fh:
+        json.dump(sweeps, fh, indent=1)
 
     with open(os.path.join(REF, "figures", "couette_flow", "linregress.csv")) as fh:
         couette = next(csv.DictReader(fh))

@@ -1,0 +1,74 @@
+"""Physics checks of the device step against closed forms and the reference's
+published numbers (FP64 unless stated; tolerances in each test)."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.mark.parametrize("storage,rel", [("f64", 3e-3), ("f32", 3e-3)])
+def test_shear_wave_viscosity_300(storage, rel):
+    """300x300 shear wave, eps 0.05, omega 1, 3000 steps: fitted
+    nu = (1/omega - 1/2)/3 = 1/6."""
+    from lattice_boltzmann_parallel_solver_amd import experiments as E
+
+    nu_fit, nu_th, t, amp = E.shear_wave_viscosity(300, 300, 1.0, 0.05, 3000, sample_every=100, storage=storage)
+    assert nu_fit == pytest.approx(nu_th, rel=rel)
+    assert amp[-1] == pytest.approx(0.05 * np.exp(-nu_th * (2 * np.pi / 300) ** 2 * 3000), rel=rel)
+
+
+@pytest.mark.parametrize("omega", [0.5, 1.0, 1.5, 1.8])
+def test_shear_wave_viscosity_vs_omega(omega):
+    """The reference's viscosity-vs-omega sweep (experiments.py:147-223) on a
+    64x64 lattice."""
+    from lattice_boltzmann_parallel_solver_amd import experiments as E
+
+    nu_fit, nu_th, _, _ = E.shear_wave_viscosity(64, 64, omega, 0.05, 1500, sample_every=25)
+    assert nu_fit == pytest.approx(nu_th, rel=2e-2)
+
+
+def test_couette_published_fit_on_gpu():
+    """figures/couette_flow/linregress.csv, reproduced to the last digits."""
+    from lattice_boltzmann_parallel_solver_amd import experiments as E
+
+    ref = json.load(open(os.path.join(GOLDEN, "reference_csv.json")))["couette"]
+    out = E.couette_profile(*ref["lattice"], U=ref["U"], omega=ref["omega"], steps=ref["steps"])
+    assert out["slope"] == pytest.approx(ref["slope"], rel=1e-12)
+    assert out["intercept"] == pytest.approx(ref["intercept"], rel=1e-12)
+
+
+def test_poiseuille_published_areas_on_gpu():
+    """figures/poiseuille_flow/areas.csv and curve_fit.csv."""
+    from lattice_boltzmann_parallel_solver_amd import experiments as E
+
+    ref = json.load(open(os.path.join(GOLDEN, "reference_csv.json")))["poiseuille"]
+    out = E.poiseuille_profile(*ref["lattice"], delta_p=ref["delta_p"], omega=ref["omega"], steps=ref["steps"])
+    assert out["area_inlet"] == pytest.approx(ref["area_inlet"], rel=1e-13)
+    assert out["area_middle"] == pytest.approx(ref["area_middle"], rel=1e-13)
+    np.testing.assert_allclose(out["popt"], ref["popt"], rtol=1e-8)
+    assert out["max_rel_error"] < 0.01  # against the analytic parabola
+
+
+@pytest.mark.parametrize("storage,tol", [("f64", 2e-4), ("f32", 2e-4)])
+def test_couette_2048_startup_matches_fourier_series(storage, tol):
+    """2048x2048 Couette (lid u = 0.05, omega 1) after 20k steps against the
+    analytic start-up profile; tol is the max |u - u_analytic| / U."""
+    from lattice_boltzmann_parallel_solver_amd import experiments as E
+
+    out = E.couette_profile(2048, 2048, U=0.05, omega=1.0, steps=20000, storage=storage)
+    assert out["max_abs_error"] / 0.05 < tol
+
+
+def test_poiseuille_steady_profile_512x64():
+    """Pressure-driven channel run to steady state against the parabola."""
+    from lattice_boltzmann_parallel_solver_amd import experiments as E
+
+    out = E.poiseuille_profile(512, 64, delta_p=1e-3, omega=1.5, steps=80000)
+    assert out["max_rel_error"] < 0.02
+    assert out["ratio"] == pytest.approx(1.0, abs=5e-3)
