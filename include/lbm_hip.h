@@ -141,6 +141,23 @@ int lbm_steps_done(const lbm_ctx *ctx, int64_t *steps);
  * fastest measured on MI355X for every storage. */
 int lbm_set_launch(lbm_ctx *ctx, int cells_per_thread, int xcd_remap, int nt_stores);
 
+/* ---- batches: many lattices advanced by one launch per step ------------- */
+/* A batch groups contexts (same device, same storage) so that one kernel
+ * launch per step advances all of them; it replaces the reference's one
+ * simulation after another (main.py:150-207) for many small lattices.  Each
+ * context keeps its own boundary program, probe and state, and stays usable
+ * on its own between batch runs; destroy the batch before its contexts. */
+typedef struct lbm_batch lbm_batch;
+/* omegas[i] is the relaxation of ctxs[i]. */
+int lbm_batch_create(lbm_batch **out, lbm_ctx *const *ctxs, const double *omegas, int n);
+int lbm_batch_destroy(lbm_batch *batch);
+/* Nodes per thread (1 or 2) and whether step pairs replay from a captured
+ * HIP graph (1) or are launched one by one (0).  Default {1, 1}. */
+int lbm_batch_set_launch(lbm_batch *batch, int nodes_per_thread, int use_graph);
+/* Advance every context of the batch nsteps reference steps (contexts still
+ * holding a loaded state start with their first collision).  Synchronous. */
+int lbm_batch_run(lbm_batch *batch, int64_t nsteps);
+
 /* ---- stateless drop-ins on device arrays (AoS, C order) ----------------- */
 /* streaming(), lattice_boltzmann_method.py:140-159: periodic shift, bit-exact. */
 int lbm_stream(const void *f_in, void *f_out, int nx, int ny, int dtype, void *stream);

@@ -9,8 +9,9 @@ Modules mirroring the reference's ``src/``:
 Device-resident API:
   solver.Lattice            one lattice on one GPU, fused step, probes, sums
   solver.BoundaryProgram    compiled boundary descriptors
+  solver.Batch              many lattices advanced by one launch per step
 """
 
-from .solver import BoundaryProgram, Lattice  # noqa: F401
+from .solver import Batch, BoundaryProgram, Lattice  # noqa: F401
 
-__all__ = ["BoundaryProgram", "Lattice"]
+__all__ = ["Batch", "BoundaryProgram", "Lattice"]

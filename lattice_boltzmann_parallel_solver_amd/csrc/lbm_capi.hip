@@ -17,6 +17,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "lbm_launch.h"
 
@@ -486,6 +487,253 @@ int lbm_steps_done(const lbm_ctx *c, int64_t *steps) {
         return LBM_ERR_ARG;
     }
     *steps = c->steps;
+    return LBM_OK;
+}
+
+// ---- batches ---------------------------------------------------------------
+
+}  // extern "C"
+
+struct lbm_batch {
+    std::vector<lbm_ctx *> ctxs;
+    std::vector<double> omegas;
+    int device = 0, storage = 0;
+    hipStream_t stream = nullptr;
+    int v = 1;
+    bool use_graph = true;
+    BatchJob *d_jobs[2] = {nullptr, nullptr};
+    uint32_t *d_block_job = nullptr;
+    size_t block_cap = 0;
+    unsigned nblocks = 0;
+    int64_t *d_step = nullptr;
+    unsigned *d_done = nullptr;
+    int64_t step_host = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    unsigned graph_blocks = 0;
+    int graph_v = 0;
+};
+
+namespace {
+
+void drop_graph(lbm_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_blocks = 0;
+}
+
+void release_batch(lbm_batch *b) {
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    (void)hipFree(b->d_jobs[0]);
+    (void)hipFree(b->d_jobs[1]);
+    (void)hipFree(b->d_block_job);
+    (void)hipFree(b->d_step);
+    (void)hipFree(b->d_done);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_batch_create(lbm_batch **out, lbm_ctx *const *ctxs, const double *omegas, int n) {
+    if (!out || n < 1 || !ctxs || !omegas) {
+        set_error("lbm_batch_create: need out, n >= 1 contexts and their omegas");
+        return LBM_ERR_ARG;
+    }
+    *out = nullptr;
+    for (int j = 0; j < n; j++) {
+        if (!ctxs[j]) {
+            set_error("lbm_batch_create: context " + std::to_string(j) + " is NULL");
+            return LBM_ERR_ARG;
+        }
+        if (ctxs[j]->device != ctxs[0]->device || ctxs[j]->storage != ctxs[0]->storage) {
+            set_error("lbm_batch_create: all contexts must share device and storage");
+            return LBM_ERR_ARG;
+        }
+        if (!(omegas[j] > 0.0 && omegas[j] < 2.0)) {
+            set_error("lbm_batch_create: omega must satisfy 0 < omega < 2");
+            return LBM_ERR_ARG;
+        }
+    }
+    auto *b = new (std::nothrow) lbm_batch();
+    if (!b) {
+        set_error("lbm_batch_create: out of host memory");
+        return LBM_ERR_NOMEM;
+    }
+    b->ctxs.assign(ctxs, ctxs + n);
+    b->omegas.assign(omegas, omegas + n);
+    b->device = ctxs[0]->device;
+    b->storage = ctxs[0]->storage;
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&b->d_jobs[0], sizeof(BatchJob) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->d_jobs[1], sizeof(BatchJob) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->d_step, sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc(&b->d_done, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(b->d_step, 0, sizeof(int64_t));
+    if (e == hipSuccess) e = hipMemset(b->d_done, 0, sizeof(unsigned));
+    if (e != hipSuccess) {
+        set_error(std::string("lbm_batch_create: ") + hipGetErrorString(e));
+        release_batch(b);
+        delete b;
+        return LBM_ERR_HIP;
+    }
+    *out = b;
+    return LBM_OK;
+}
+
+int lbm_batch_destroy(lbm_batch *b) {
+    if (!b) return LBM_OK;
+    release_batch(b);
+    delete b;
+    return LBM_OK;
+}
+
+int lbm_batch_set_launch(lbm_batch *b, int v, int use_graph) {
+    if (!b || (v != 1 && v != 2)) {
+        set_error("lbm_batch_set_launch: nodes_per_thread must be 1 or 2");
+        return LBM_ERR_ARG;
+    }
+    b->v = v;
+    b->use_graph = use_graph != 0;
+    return LBM_OK;
+}
+
+int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
+    if (!b) {
+        set_error("lbm_batch_run: batch is NULL");
+        return LBM_ERR_ARG;
+    }
+    if (nsteps < 0) {
+        set_error("lbm_batch_run: nsteps must be >= 0");
+        return LBM_ERR_ARG;
+    }
+    LBM_HIP_CHECK(hipSetDevice(b->device), "lbm_batch_run");
+    const int n = (int)b->ctxs.size();
+    for (int j = 0; j < n; j++) {
+        lbm_ctx *c = b->ctxs[j];
+        if (c->phase == kEmpty) {
+            set_error("lbm_batch_run: context " + std::to_string(j) + " has no state loaded");
+            return LBM_ERR_STATE;
+        }
+        LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_batch_run: context stream");
+    }
+    if (nsteps == 0) return LBM_OK;
+    // A context holding a loaded step input starts with its first collision,
+    // which is one step; if any does, every other context takes one step on
+    // its own too, so that the batched launches advance all of them alike.
+    bool any_pending = false;
+    for (lbm_ctx *c : b->ctxs) any_pending |= c->phase == kPending;
+    if (any_pending) {
+        for (int j = 0; j < n; j++) {
+            lbm_ctx *c = b->ctxs[j];
+            if (c->phase == kPending) {
+                LBM_HIP_CHECK(launch_collide_first(c->storage, c->g, c->stage_f, c->stage_rho, c->stage_u, c->st,
+                                                   c->cur, c->bc.view, b->omegas[j], b->stream),
+                              "lbm_batch_run: first collision");
+                c->phase = kRunning;
+            } else {
+                const int64_t slot = c->probe.cell >= 0 ? c->steps - c->probe_base - 1 : -1;
+                LBM_HIP_CHECK(launch_fused(c->storage, c->g, c->st, c->cur, c->bc.view, b->omegas[j], c->probe, slot,
+                                           nullptr, c->cfg, b->stream, nullptr),
+                              "lbm_batch_run: aligning step");
+                c->cur ^= 1;
+            }
+            c->steps++;
+        }
+        nsteps--;
+    }
+
+    // job tables for the two step parities of this run
+    std::vector<BatchJob> jobs[2];
+    std::vector<uint32_t> block_job;
+    const int per_block = batch_block_nodes(b->v);
+    for (int p = 0; p < 2; p++) jobs[p].resize(n);
+    for (int j = 0; j < n; j++) {
+        lbm_ctx *c = b->ctxs[j];
+        const uint32_t nb = (uint32_t)(((int64_t)c->g.N + per_block - 1) / per_block);
+        const uint32_t block0 = (uint32_t)block_job.size();
+        block_job.insert(block_job.end(), nb, (uint32_t)j);
+        for (int p = 0; p < 2; p++) {
+            BatchJob &d = jobs[p][j];
+            const int src = c->cur ^ p;
+            d.src = c->st.plane[src];
+            d.dst = c->st.plane[src ^ 1];
+            d.side_prev = c->st.side[src];
+            d.side_next = c->st.side[src ^ 1];
+            d.g = c->g;
+            d.bc = c->bc.view;
+            d.omega = b->omegas[j];
+            d.block0 = block0;
+            if (c->probe.cell >= 0) {
+                d.probe_cell = c->probe.cell;
+                d.probe_ring = c->probe.ring;
+                d.probe_cap = c->probe.capacity;
+                d.probe_offset = c->steps - c->probe_base - 1 - b->step_host;
+            }
+        }
+    }
+    const unsigned nblocks = (unsigned)block_job.size();
+    if (nblocks > b->block_cap) {
+        drop_graph(b);
+        (void)hipFree(b->d_block_job);
+        b->d_block_job = nullptr;
+        LBM_HIP_CHECK(hipMalloc(&b->d_block_job, sizeof(uint32_t) * nblocks), "lbm_batch_run: block table");
+        b->block_cap = nblocks;
+    }
+    LBM_HIP_CHECK(hipMemcpyAsync(b->d_block_job, block_job.data(), sizeof(uint32_t) * nblocks,
+                                 hipMemcpyHostToDevice, b->stream), "lbm_batch_run: block table");
+    for (int p = 0; p < 2; p++)
+        LBM_HIP_CHECK(hipMemcpyAsync(b->d_jobs[p], jobs[p].data(), sizeof(BatchJob) * n, hipMemcpyHostToDevice,
+                                     b->stream), "lbm_batch_run: job table");
+    b->nblocks = nblocks;
+
+    // step pairs replayed from the graph, then the odd one
+    int64_t left = nsteps;
+    const int64_t pairs = left / 2;
+    if (b->use_graph && pairs > 0) {
+        if (!b->exec || b->graph_blocks != nblocks || b->graph_v != b->v) {
+            drop_graph(b);
+            LBM_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+            hipError_t e0 = launch_fused_batch(b->storage, b->v, b->d_jobs[0], b->d_block_job, nblocks, b->d_step,
+                                               b->d_done, b->stream);
+            hipError_t e1 = launch_fused_batch(b->storage, b->v, b->d_jobs[1], b->d_block_job, nblocks, b->d_step,
+                                               b->d_done, b->stream);
+            hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
+            if (e0 != hipSuccess || e1 != hipSuccess || ec != hipSuccess) {
+                set_error("lbm_batch_run: graph capture failed");
+                drop_graph(b);
+                return LBM_ERR_HIP;
+            }
+            LBM_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+            b->graph_blocks = nblocks;
+            b->graph_v = b->v;
+        }
+        for (int64_t k = 0; k < pairs; k++) LBM_HIP_CHECK(hipGraphLaunch(b->exec, b->stream), "graph launch");
+        left -= 2 * pairs;
+    }
+    for (int64_t k = 0; left > 0; k++, left--) {
+        const int p = (int)((nsteps - left) & 1);
+        LBM_HIP_CHECK(launch_fused_batch(b->storage, b->v, b->d_jobs[p], b->d_block_job, nblocks, b->d_step,
+                                         b->d_done, b->stream),
+                      "batched step kernel");
+    }
+    b->step_host += nsteps;
+    for (int j = 0; j < n; j++) {
+        lbm_ctx *c = b->ctxs[j];
+        c->cur ^= (int)(nsteps & 1);
+        c->steps += nsteps;
+        if (c->probe.cell >= 0 && c->steps > c->probe_base)
+            LBM_HIP_CHECK(launch_probe_cell(c->storage, c->g, c->st, c->cur, c->bc.view, c->probe,
+                                            c->steps - c->probe_base - 1, b->stream),
+                          "probe kernel");
+    }
+    LBM_HIP_CHECK(hipStreamSynchronize(b->stream), "lbm_batch_run");
     return LBM_OK;
 }
 

@@ -560,6 +560,59 @@ __global__ __launch_bounds__(kBlock) void k_fused16(FusedParams<S> p) {
 }
 
 // ---------------------------------------------------------------------------
+// the fused step over a batch of independent lattices
+//
+// Many small lattices (tens to hundreds of thousands of nodes) would each
+// leave most of the 256 CUs idle and pay a launch per step; here one launch
+// advances all of them.  Block b belongs to job block_job[b]; the job's
+// descriptor gives its buffers, geometry, boundary tables and omega.  A
+// device step counter, advanced by a one-thread kernel after each step, lets
+// the probes of a graph-replayed launch sequence find their ring slot.
+// ---------------------------------------------------------------------------
+template <class S, int V>
+__global__ __launch_bounds__(kBlock) void k_fused_batch(const BatchJob *__restrict__ jobs,
+                                                        const uint32_t *__restrict__ block_job,
+                                                        const int64_t *__restrict__ step) {
+    using C = typename Io<S>::C;
+    const BatchJob &d = jobs[block_job[blockIdx.x]];
+    const Geom &g = d.g;
+    const S *src = static_cast<const S *>(d.src);
+    S *dst = static_cast<S *>(d.dst);
+    const C *side_prev = static_cast<const C *>(d.side_prev);
+    C *side_next = static_cast<C *>(d.side_next);
+    const int64_t s_now = *step;
+    const uint32_t lb = blockIdx.x - d.block0;
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+        const uint32_t n = (lb * V + k) * kBlock + threadIdx.x;
+        if (n < g.N) {
+            const uint32_t x = row_of(g, n), y = n - x * g.ly;
+            C v[9];
+            pull<S>(src, g, x, y, v);
+            uint32_t flags = 0;
+            if (d.bc.chunk) {
+                const uint32_t chunk = __builtin_amdgcn_readfirstlane(n >> 6);
+                if (d.bc.chunk[chunk]) flags = apply_class<S>(d.bc, src, side_prev, g, n, x, y, v);
+            }
+            NodeOut<S> o;
+            collide_node<S>(v, d.omega, o);
+            store_node<S>(dst, side_next, g, d.bc, x, y, flags, v, o, false);
+            if ((int64_t)n == d.probe_cell) {
+                const int64_t slot = s_now + d.probe_offset;
+                if (slot >= 0) {
+                    double *out = d.probe_ring + 3 * (slot % d.probe_cap);
+                    out[0] = o.rho;
+                    out[1] = (double)o.ux;
+                    out[2] = (double)o.uy;
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_next_step(int64_t *step) { *step += 1; }
+
+// ---------------------------------------------------------------------------
 // first collision (from the AoS FP64 step input)
 // ---------------------------------------------------------------------------
 template <class S>
@@ -853,6 +906,32 @@ hipError_t fused_typed(const Geom &g, const StateView &st, int src, const BcView
 }
 
 }  // namespace
+
+int batch_block_nodes(int nodes_per_thread) { return kBlock * nodes_per_thread; }
+
+hipError_t launch_fused_batch(int storage, int v, const BatchJob *jobs, const uint32_t *block_job, unsigned nblocks,
+                              int64_t *step, unsigned *done, hipStream_t s) {
+    if (nblocks == 0) return hipSuccess;
+#define LBM_BATCH(T, VV) hipLaunchKernelGGL((k_fused_batch<T, VV>), dim3(nblocks), dim3(kBlock), 0, s, jobs, block_job, \
+                                            step)
+    if (v == 1) {
+        if (storage == LBM_STORE_F64) LBM_BATCH(double, 1);
+        else if (storage == LBM_STORE_F32) LBM_BATCH(float, 1);
+        else LBM_BATCH(__half, 1);
+    } else if (v == 2) {
+        if (storage == LBM_STORE_F64) LBM_BATCH(double, 2);
+        else if (storage == LBM_STORE_F32) LBM_BATCH(float, 2);
+        else LBM_BATCH(__half, 2);
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef LBM_BATCH
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    (void)done;
+    hipLaunchKernelGGL(k_next_step, dim3(1), dim3(1), 0, s, step);
+    return hipGetLastError();
+}
 
 int vec16_nodes(int storage) { return storage == LBM_STORE_F64 ? 2 : (storage == LBM_STORE_F32 ? 4 : 8); }
 

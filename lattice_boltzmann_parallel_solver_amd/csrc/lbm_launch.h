@@ -56,6 +56,30 @@ int fused_blocks(const Geom &g, const LaunchCfg &cfg);
 // Nodes per thread of the 16-byte kernel for a storage type.
 int vec16_nodes(int storage);
 
+// One lattice of a batch, with its buffers already resolved for the step
+// parity of the launch (src -> dst).
+struct BatchJob {
+    const void *src = nullptr;
+    void *dst = nullptr;
+    const void *side_prev = nullptr;
+    void *side_next = nullptr;
+    Geom g;
+    BcView bc;
+    double omega = 1.0;
+    int64_t probe_cell = -1;
+    double *probe_ring = nullptr;
+    int64_t probe_cap = 0;
+    int64_t probe_offset = 0;  // ring slot = batch step counter + probe_offset
+    uint32_t block0 = 0;       // first block of the job in the batch grid
+    uint32_t pad = 0;
+};
+
+// One fused step of every job of a batch (block b runs job block_job[b]),
+// followed by a one-thread kernel that advances the batch step counter *step.
+hipError_t launch_fused_batch(int storage, int nodes_per_thread, const BatchJob *jobs, const uint32_t *block_job,
+                              unsigned nblocks, int64_t *step, unsigned *done, hipStream_t s);
+int batch_block_nodes(int nodes_per_thread);
+
 // First collision from AoS FP64 (f, rho, u) into planes[dst] / side[dst].
 hipError_t launch_collide_first(int storage, const Geom &g, const double *f, const double *rho,
                                 const double *u, const StateView &st, int dst, const BcView &bc,

@@ -259,3 +259,45 @@ class Lattice:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Several lattices advanced together, one kernel launch per step
+    (lbm_batch_*).  Lattices must share device and storage; omegas[i] is the
+    relaxation of lattices[i].  Close the batch before its lattices."""
+
+    def __init__(self, lattices: Sequence[Lattice], omegas: Sequence[float], nodes_per_thread: int = 1,
+                 use_graph: bool = True):
+        if len(lattices) != len(omegas) or not lattices:
+            raise ValueError("need one omega per lattice and at least one lattice")
+        L = _lib.load()
+        self.lattices = list(lattices)
+        handles = (_c_p * len(lattices))(*[lat._h.value for lat in lattices])
+        oms = (ctypes.c_double * len(lattices))(*[float(o) for o in omegas])
+        h = _c_p()
+        check(L.lbm_batch_create(ctypes.byref(h), handles, oms, len(lattices)))
+        self._h = h
+        self.set_launch(nodes_per_thread, use_graph)
+
+    def set_launch(self, nodes_per_thread: int = 1, use_graph: bool = True) -> None:
+        check(_lib.load().lbm_batch_set_launch(self._h, int(nodes_per_thread), int(bool(use_graph))))
+
+    def run(self, steps: int) -> None:
+        """Advance every lattice `steps` reference steps (synchronous)."""
+        check(_lib.load().lbm_batch_run(self._h, int(steps)))
+
+    @property
+    def nodes(self) -> int:
+        return sum(l.lx * l.ly for l in self.lattices)
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.load().lbm_batch_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

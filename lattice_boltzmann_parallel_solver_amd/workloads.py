@@ -1,0 +1,201 @@
+"""Batched channel workloads, the sequential baseline and the launch autotuner.
+
+The reference runs its simulations one after another (main.py:150-207, one
+x_strouhal call per Reynolds number / length / blockage ratio).  Here many
+uneven channels advance together: ``run_batched`` puts them in one Batch (one
+kernel launch per step for all of them, step pairs replayed from a HIP
+graph), ``run_sequential`` solves them one at a time with the plain
+single-lattice path, for comparison.
+
+``autotune_lattice`` / ``autotune_batch`` pick the launch shape of the fused
+step per size bucket by timing the candidates on scratch lattices of the same
+shape and boundary (the real state is never advanced by tuning); results are
+cached per (storage, size bucket).
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .boundary_utils import von_karman_boundary_conditions
+from .initial_values import density_1_velocity_x_u0_velocity_y_0_initial
+from .solver import Batch, Lattice
+
+
+@dataclass
+class ChannelJob:
+    """One channel: plate of blockage d/ly at x = lx//4 (+1 y offset), rigid
+    walls, inlet u = 0.1."""
+    lx: int
+    ly: int
+    plate: int
+    reynolds: float
+    u_in: float = 0.1
+
+    @property
+    def nu(self) -> float:
+        return self.u_in * self.plate / self.reynolds
+
+    @property
+    def omega(self) -> float:
+        return 1.0 / (3.0 * self.nu + 0.5)
+
+    @property
+    def nodes(self) -> int:
+        return self.lx * self.ly
+
+
+def channel_jobs(n: int = 256, seed: int = 0, nx_range=(200, 1200), blockage=(0.1, 0.4),
+                 reynolds=(40.0, 200.0)) -> List[ChannelJob]:
+    """Seeded draw: nx ~ U[200, 1200], ny = round(3 nx / 7), B ~ U[0.1, 0.4],
+    d = round(B ny), Re ~ U[40, 200]."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(n):
+        lx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        ly = int(round(3 * lx / 7))
+        d = max(2, int(round(rng.uniform(*blockage) * ly)))
+        re = float(rng.uniform(*reynolds))
+        jobs.append(ChannelJob(lx, ly, d, re))
+    return jobs
+
+
+def build_lattice(job: ChannelJob, storage: str = "f32", device: int = 0) -> Lattice:
+    lat = Lattice(job.lx, job.ly,
+                  boundary=von_karman_boundary_conditions(job.lx, job.ly, 1.0, job.u_in, job.plate, walls=True,
+                                                          y_offset=1),
+                  storage=storage, device=device)
+    rho, u = density_1_velocity_x_u0_velocity_y_0_initial((job.lx, job.ly), job.u_in)
+    lat.set_equilibrium(rho, u)
+    return lat
+
+
+def _sync_all(lattices):
+    for lat in lattices:
+        lat.synchronize()
+
+
+def run_sequential(jobs: Sequence[ChannelJob], steps: int, storage: str = "f32", tune: bool = False):
+    """One channel at a time (create, run `steps`, finish), as the reference's
+    sweeps do.  Returns (seconds, aggregate MLUPS)."""
+    total = 0.0
+    for job in jobs:
+        lat = build_lattice(job, storage)
+        if tune:
+            autotune_lattice(lat, job.omega)
+        lat.synchronize()
+        t0 = time.perf_counter()
+        lat.run(job.omega, steps)
+        lat.synchronize()
+        total += time.perf_counter() - t0
+        lat.close()
+    return total, sum(j.nodes for j in jobs) * steps / total / 1e6
+
+
+def run_batched(jobs: Sequence[ChannelJob], steps: int, storage: str = "f32", nodes_per_thread: Optional[int] = None,
+                use_graph: bool = True, chunk: int = 1000, keep: bool = False):
+    """All channels in one Batch.  Returns (seconds, aggregate MLUPS, lattices
+    or None)."""
+    lats = [build_lattice(j, storage) for j in jobs]
+    oms = [j.omega for j in jobs]
+    if nodes_per_thread is None:
+        nodes_per_thread = autotune_batch(jobs, storage)
+    b = Batch(lats, oms, nodes_per_thread=nodes_per_thread, use_graph=use_graph)
+    b.run(1)  # first collisions, tables, graph capture
+    _sync_all(lats)
+    t0 = time.perf_counter()
+    done = 1
+    while done < steps:
+        k = min(chunk, steps - done)
+        b.run(k)
+        done += k
+    sec = time.perf_counter() - t0
+    b.close()
+    mlups = sum(j.nodes for j in jobs) * (steps - 1) / sec / 1e6
+    if not keep:
+        for lat in lats:
+            lat.close()
+        lats = None
+    return sec, mlups, lats
+
+
+# ----------------------------------------------------------------------------- autotuning
+_LATTICE_CACHE: Dict[Tuple[str, int], Tuple[int, bool, bool]] = {}
+_BATCH_CACHE: Dict[Tuple[str, int], int] = {}
+
+
+def size_bucket(nodes: int) -> int:
+    """Half-octave bucket of a node count."""
+    return int(round(2 * math.log2(max(1, nodes))))
+
+
+def lattice_candidates(ly: int, storage: str):
+    vec = {"f64": 2, "f32": 4, "f16": 8}[storage]
+    cands = [(v, xcd, False) for v in (1, 2, 4) for xcd in (False, True)]
+    if ly % vec == 0:
+        cands += [(0, False, False), (0, True, False)]
+    return cands
+
+
+def _time_run(lat, omega, steps) -> float:
+    lat.synchronize()
+    t0 = time.perf_counter()
+    lat.run(omega, steps)
+    lat.synchronize()
+    return time.perf_counter() - t0
+
+
+def autotune_lattice(lat: Lattice, omega: float, steps: int = 30) -> Tuple[int, bool, bool]:
+    """Fastest launch shape for a lattice of this storage and size bucket,
+    timed on a scratch lattice with the same shape and boundary; applied to
+    `lat` and cached."""
+    key = (lat.storage, size_bucket(lat.lx * lat.ly))
+    best = _LATTICE_CACHE.get(key)
+    if best is None:
+        scratch = Lattice(lat.lx, lat.ly, boundary=lat._program, storage=lat.storage, device=lat.device)
+        rho = np.ones((lat.lx, lat.ly))
+        u = np.zeros((lat.lx, lat.ly, 2))
+        u[..., 0] = 0.05
+        scratch.set_equilibrium(rho, u)
+        scratch.run(omega, 2)
+        timings = []
+        for cand in lattice_candidates(lat.ly, lat.storage):
+            scratch.set_launch(*cand)
+            _time_run(scratch, omega, 3)
+            timings.append((min(_time_run(scratch, omega, steps) for _ in range(2)), cand))
+        scratch.close()
+        best = min(timings)[1]
+        _LATTICE_CACHE[key] = best
+    lat.set_launch(*best)
+    return best
+
+
+def autotune_batch(jobs: Sequence[ChannelJob], storage: str = "f32", sample: int = 16, steps: int = 20) -> int:
+    """Nodes per thread of the batched kernel (1 or 2) for this storage and
+    mean job size, timed on a batch of scratch lattices of the first
+    `sample` job shapes; cached."""
+    key = (storage, size_bucket(int(np.mean([j.nodes for j in jobs]))))
+    best = _BATCH_CACHE.get(key)
+    if best is None:
+        sub = list(jobs[:sample])
+        lats = [build_lattice(j, storage) for j in sub]
+        b = Batch(lats, [j.omega for j in sub])
+        b.run(2)
+        timings = []
+        for v in (1, 2):
+            b.set_launch(v, True)
+            b.run(2)
+            t0 = time.perf_counter()
+            b.run(steps)
+            timings.append((time.perf_counter() - t0, v))
+        b.close()
+        for lat in lats:
+            lat.close()
+        best = min(timings)[1]
+        _BATCH_CACHE[key] = best
+    return best

@@ -1,0 +1,108 @@
+"""Batched step (one launch per step for many lattices) against individual
+lattices run one by one: FP64 bit for bit, including graph replays
+interleaved with eager work, mixed pending/running contexts and probes."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def make_jobs(storage="f64", seed=0, n=6):
+    """Uneven channels plus a Couette and a periodic lattice."""
+    from lattice_boltzmann_parallel_solver_amd import Lattice
+    from lattice_boltzmann_parallel_solver_amd.boundary_utils import (couette_flow_boundary_conditions,
+                                                                      von_karman_boundary_conditions)
+    from lattice_boltzmann_parallel_solver_amd.initial_values import (density_1_velocity_x_u0_velocity_y_0_initial,
+                                                                      sinusoidal_velocity_x)
+
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(n):
+        lx = int(rng.integers(40, 160))
+        ly = int(round(3 * lx / 7))
+        d = max(2, int(round(rng.uniform(0.1, 0.4) * ly)))
+        re = rng.uniform(40, 200)
+        om = 1.0 / (3 * 0.1 * d / re + 0.5)
+        bc = von_karman_boundary_conditions(lx, ly, 1.0, 0.1, d, walls=True, y_offset=1)
+        rho, u = density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), 0.1)
+        jobs.append((lx, ly, bc, rho, u, om))
+    rho = np.ones((33, 21))
+    jobs.append((33, 21, couette_flow_boundary_conditions(33, 21, 0.05, 1.0), rho, np.zeros((33, 21, 2)), 1.0))
+    rho, u = sinusoidal_velocity_x((17, 64), 0.05)
+    jobs.append((17, 64, None, rho, u, 0.7))
+    lats = []
+    for lx, ly, bc, rho, u, om in jobs:
+        lat = Lattice(lx, ly, boundary=bc, storage=storage)
+        lat.set_equilibrium(rho, u)
+        lats.append(lat)
+    return lats, [j[5] for j in jobs]
+
+
+@pytest.mark.parametrize("v,graph", [(1, True), (2, True), (1, False)])
+def test_batch_equals_individual_runs_fp64(v, graph):
+    from lattice_boltzmann_parallel_solver_amd import Batch
+
+    lats, oms = make_jobs()
+    ref, _ = make_jobs()
+    b = Batch(lats, oms, nodes_per_thread=v, use_graph=graph)
+    for chunk in (7, 10, 1, 24):  # odd and even chunks, graph replays between eager reads
+        b.run(chunk)
+        for lat, r, om in zip(lats, ref, oms):
+            r.run(om, chunk)
+            got, want = lat.state(), r.state()
+            for a, c in zip(got, want):
+                assert np.array_equal(a, c)
+    b.close()
+
+
+def test_batch_with_running_and_pending_contexts_and_probes():
+    from lattice_boltzmann_parallel_solver_amd import Batch
+
+    lats, oms = make_jobs(seed=3, n=3)
+    ref, _ = make_jobs(seed=3, n=3)
+    lats[0].run(oms[0], 5)
+    ref[0].run(oms[0], 5)
+    for lat, r in zip(lats, ref):
+        lat.set_probe(lat.lx // 2, lat.ly // 2, 100)
+        r.set_probe(r.lx // 2, r.ly // 2, 100)
+    b = Batch(lats, oms)
+    b.run(12)
+    b.run(9)
+    for lat, r, om in zip(lats, ref, oms):
+        r.run(om, 21)
+        assert lat.steps == r.steps
+        assert np.array_equal(lat.probe(), r.probe())
+        for a, c in zip(lat.state(), r.state()):
+            assert np.array_equal(a, c)
+    b.close()
+
+
+def test_batch_fp32_matches_individual():
+    from lattice_boltzmann_parallel_solver_amd import Batch
+
+    lats, oms = make_jobs(storage="f32", seed=5)
+    ref, _ = make_jobs(storage="f32", seed=5)
+    b = Batch(lats, oms)
+    b.run(40)
+    for lat, r, om in zip(lats, ref, oms):
+        r.run(om, 40)
+        for a, c in zip(lat.state(), r.state()):
+            np.testing.assert_allclose(a, c, rtol=0, atol=1e-6)
+    b.close()
+
+
+def test_batch_errors():
+    from lattice_boltzmann_parallel_solver_amd import Batch, Lattice
+    from lattice_boltzmann_parallel_solver_amd._lib import LbmError
+
+    a = Lattice(8, 8, storage="f64")
+    c = Lattice(8, 8, storage="f32")
+    with pytest.raises(LbmError, match="share"):
+        Batch([a, c], [1.0, 1.0])
+    with pytest.raises(LbmError, match="omega"):
+        Batch([a], [2.0])
+    b = Batch([a], [1.0])
+    with pytest.raises(LbmError, match="no state"):
+        b.run(3)
+    b.close()
