@@ -12,6 +12,10 @@ roofline   : HBM bound; achieved = algorithmic bytes per launch (18 population
              launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated
              (profiles/fused_step_8192x2048_hbm_traffic.json).
 fp64, f16  : the same workload with FP64 populations / FP16 deviation storage.
+latency    : p50/p99 time-to-result of the continuous-batching server on a seeded
+             Poisson job stream (scaled serving mix, stated in the line).
+batching   : aggregate MLUPS of seeded uneven channels advanced by one launch per
+             step versus the same channels solved one at a time.
 cpu_baseline: the oracle (numpy FP64, one core) on a few steps of the same workload.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
@@ -75,6 +79,33 @@ def time_steps(lat, steps, warmup, torch):
     return start.elapsed_time(end)  # ms
 
 
+def serving_leg(n=24, seed=0):
+    """Continuous batching on a seeded Poisson stream (50 jobs/s): 60 % von
+    Karman 2048x512 channels, 20 % Couette, 20 % Poiseuille on squares of side
+    U[128, 1024], budgets log-uniform in [5k, 20k] steps, FP32."""
+    from lattice_boltzmann_parallel_solver_amd.serving import Server, c5_jobs, summarize
+
+    arrivals = c5_jobs(n, seed=seed, vk_size=(2048, 512), square=(128, 1024), budget=(5000, 20000))
+    server = Server(chunk_steps=2000)
+    recs = server.run(arrivals)
+    s = summarize(recs, server.makespan)
+    s["workload"] = (f"{n} jobs, Poisson 50/s, 60% von Karman 2048x512, 20% Couette, 20% Poiseuille "
+                     "squares U[128,1024], budgets log-uniform [5k,20k], f32, seed 0")
+    return s
+
+
+def batching_leg(n=64, steps=1000, seed=0):
+    """Seeded uneven channels (nx U[200,1200], ny = 3nx/7, blockage U[0.1,0.4],
+    Re U[40,200], FP32): one launch per step for all versus one at a time."""
+    from lattice_boltzmann_parallel_solver_amd import workloads as W
+
+    jobs = W.channel_jobs(n, seed=seed)
+    _, batched, _ = W.run_batched(jobs, steps, "f32")
+    _, sequential = W.run_sequential(jobs, steps, "f32")
+    return {"jobs": n, "steps": steps, "nodes": sum(j.nodes for j in jobs), "batched_mlups": batched,
+            "sequential_mlups": sequential, "speedup": batched / sequential}
+
+
 def cpu_baseline(steps=2):
     """numpy oracle (FP64, one core) on `steps` steps of the same workload after one warm-up step."""
     import numpy as np
@@ -104,6 +135,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=200)
     ap.add_argument("--f16-steps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -155,6 +188,8 @@ def main():
     fp64 = leg("f64", args.fp64_steps)
     f16 = leg("f16", args.f16_steps)
 
+    latency = None if args.no_serving else serving_leg()
+    batching = None if args.no_batching else batching_leg()
     cpu = None if args.no_cpu_baseline else cpu_baseline()
 
     line = {
@@ -179,7 +214,8 @@ def main():
                      "bytes_per_node": BYTES_PER_NODE[args.storage]},
         "fp64": fp64,
         "f16": f16,
-        "latency": None,
+        "latency": latency,
+        "batching": batching,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -1,0 +1,240 @@
+"""Continuous-batching serving of simulation jobs on one GPU.
+
+Jobs (von Karman channels, Couette and Poiseuille boxes of any size and step
+budget) arrive over time.  The server admits waiting jobs while their HBM
+footprint fits the budget, keeps every admitted lattice resident, and
+advances all of them together with a Batch (one kernel launch per step for
+the whole active set), in chunks.  When jobs finish, their results are read,
+their memory is freed and waiting jobs are admitted at the next chunk
+boundary, so new work overlaps with work already on the device.  The reference
+ran one simulation after another (main.py:150-207).
+
+Reported per job: arrival, admission and completion times (time-to-result is
+completion - arrival), the result (probe trace and Strouhal numbers for a
+channel, the u_x profile for Couette / Poiseuille), and whether the FP64
+moment sums stayed finite.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from ._lib import LbmError
+from .boundary_utils import (couette_flow_boundary_conditions, poiseuille_flow_boundary_conditions,
+                             von_karman_boundary_conditions)
+from .experiments import speed, strouhal_interpolated, strouhal_reference
+from .initial_values import density_1_velocity_0_initial, density_1_velocity_x_u0_velocity_y_0_initial
+from .solver import Batch, Lattice
+
+STORAGE_BYTES = {"f64": 8, "f32": 4, "f16": 2}
+
+
+@dataclass
+class JobSpec:
+    kind: str                 # "von_karman", "couette" or "poiseuille"
+    lx: int
+    ly: int
+    steps: int
+    storage: str = "f32"
+    plate: int = 400          # von Karman
+    reynolds: float = 100.0   # von Karman
+    u_in: float = 0.1         # von Karman inlet / Couette lid
+    omega: float = 1.0        # Couette / Poiseuille relaxation
+    delta_p: float = 1e-3     # Poiseuille
+
+    @property
+    def nodes(self) -> int:
+        return self.lx * self.ly
+
+    @property
+    def relaxation(self) -> float:
+        if self.kind == "von_karman":
+            return 1.0 / (3.0 * self.u_in * self.plate / self.reynolds + 0.5)
+        return self.omega
+
+    def footprint(self) -> int:
+        """Device bytes of the job's context: two state buffers, FP64 staging,
+        boundary codes, probe ring."""
+        s = STORAGE_BYTES[self.storage]
+        probe = 24 * self.steps if self.kind == "von_karman" else 0
+        return self.nodes * (18 * s + 96 + 2) + probe + (1 << 20)
+
+    def boundary(self):
+        if self.kind == "von_karman":
+            return von_karman_boundary_conditions(self.lx, self.ly, 1.0, self.u_in, self.plate, walls=True,
+                                                  y_offset=1)
+        if self.kind == "couette":
+            return couette_flow_boundary_conditions(self.lx, self.ly, self.u_in, 1.0)
+        if self.kind == "poiseuille":
+            drho = 3 * self.delta_p
+            return poiseuille_flow_boundary_conditions(self.lx, self.ly, (1 + drho / 2) / 3, (1 - drho / 2) / 3)
+        raise ValueError(self.kind)
+
+    def start(self, device: int = 0, storage: Optional[str] = None) -> Lattice:
+        lat = Lattice(self.lx, self.ly, boundary=self.boundary(), storage=storage or self.storage, device=device)
+        if self.kind == "von_karman":
+            rho, u = density_1_velocity_x_u0_velocity_y_0_initial((self.lx, self.ly), self.u_in)
+            lat.set_probe(3 * self.lx // 4, self.ly // 2, self.steps)
+        else:
+            rho, u = density_1_velocity_0_initial((self.lx, self.ly))
+        lat.set_equilibrium(rho, u)
+        return lat
+
+    def result(self, lat: Lattice) -> dict:
+        """The job's answer, read from the device."""
+        out = {}
+        try:
+            m, jx, jy = lat.moments_sum()
+            out["mass"] = m
+            out["finite"] = True
+        except LbmError:
+            out["finite"] = False
+            return out
+        if self.kind == "von_karman":
+            samples = lat.probe()
+            out["samples"] = samples
+            start = len(samples) // 2
+            if len(samples) - start > 16:
+                out["strouhal_uy"] = strouhal_interpolated(samples[:, 2], self.plate, self.u_in, start)
+                out["strouhal_speed"] = strouhal_interpolated(speed(samples), self.plate, self.u_in, start)
+                out["strouhal_uy_fft"] = strouhal_reference(samples[:, 2], self.plate, self.u_in, start)[0]
+        else:
+            _, _, u = lat.state(want=("u",))
+            out["ux_mid"] = u[self.lx // 2, :, 0].copy()
+        return out
+
+
+@dataclass
+class JobRecord:
+    job_id: int
+    spec: JobSpec
+    arrival: float
+    admitted: float = math.nan
+    finished: float = math.nan
+    result: dict = field(default_factory=dict)
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def c5_jobs(n: int, seed: int = 0, rate: float = 50.0, vk_size=(8192, 2048), square=(128, 2048),
+            budget=(5000, 100000), storage: str = "f32") -> List[Tuple[float, JobSpec]]:
+    """Seeded serving workload: Poisson arrivals at `rate` jobs/s; 60 % von
+    Karman channels of size vk_size (plate ly/5, Re 100), 20 % Couette and 20 %
+    Poiseuille on squares with side ~ U[square]; step budgets log-uniform in
+    `budget`."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += rng.exponential(1.0 / rate)
+        steps = int(round(math.exp(rng.uniform(math.log(budget[0]), math.log(budget[1])))))
+        r = rng.random()
+        if r < 0.6:
+            lx, ly = vk_size
+            spec = JobSpec("von_karman", lx, ly, steps, storage, plate=max(2, ly // 5), reynolds=100.0)
+        else:
+            side = int(rng.integers(square[0], square[1] + 1))
+            if r < 0.8:
+                spec = JobSpec("couette", side, side, steps, storage, u_in=0.05, omega=1.0)
+            else:
+                spec = JobSpec("poiseuille", side, side, steps, storage, omega=1.5, delta_p=1e-3)
+        out.append((t, spec))
+    return out
+
+
+class Server:
+    """Continuous batching with admission by HBM footprint."""
+
+    def __init__(self, hbm_budget: int = 160 << 30, chunk_steps: int = 2000, max_active: int = 256,
+                 device: int = 0, nodes_per_thread: int = 1):
+        self.hbm_budget = int(hbm_budget)
+        self.chunk_steps = int(chunk_steps)
+        self.max_active = int(max_active)
+        self.device = device
+        self.nodes_per_thread = nodes_per_thread
+        # bring up the HIP runtime and load the kernels before the first job arrives
+        warm = Lattice(8, 8, device=device)
+        warm.set_equilibrium(np.ones((8, 8)), np.zeros((8, 8, 2)))
+        warm.run(1.0, 2)
+        warm.synchronize()
+        warm.close()
+
+    def run(self, arrivals: Sequence[Tuple[float, JobSpec]], realtime: bool = True) -> List[JobRecord]:
+        """Serve the jobs; arrival times are seconds after the call (with
+        realtime=False every job is available at once).  Returns the records in
+        submission order."""
+        records = [JobRecord(i, spec, t if realtime else 0.0) for i, (t, spec) in enumerate(arrivals)]
+        for r in records:
+            if r.spec.footprint() > self.hbm_budget:
+                raise ValueError(f"job {r.job_id} needs {r.spec.footprint()} B, more than the HBM budget")
+        incoming = deque(sorted(records, key=lambda r: r.arrival))
+        waiting: deque = deque()
+        active: Dict[int, Tuple[JobRecord, Lattice, int]] = {}  # id -> (record, lattice, steps left)
+        used = 0
+        batch: Optional[Batch] = None
+        members: Tuple[int, ...] = ()
+        t0 = time.perf_counter()
+        self.busy = 0.0
+        while incoming or waiting or active:
+            now = time.perf_counter() - t0
+            while incoming and incoming[0].arrival <= now:
+                waiting.append(incoming.popleft())
+            # admission: FIFO while the footprint fits
+            while waiting and len(active) < self.max_active and used + waiting[0].spec.footprint() <= self.hbm_budget:
+                rec = waiting.popleft()
+                lat = rec.spec.start(self.device)
+                rec.admitted = time.perf_counter() - t0
+                active[rec.job_id] = (rec, lat, rec.spec.steps)
+                used += rec.spec.footprint()
+            if not active:
+                if incoming:
+                    time.sleep(max(0.0, incoming[0].arrival - (time.perf_counter() - t0)))
+                continue
+            ids = tuple(sorted(active))
+            if ids != members:
+                if batch is not None:
+                    batch.close()
+                batch = Batch([active[i][1] for i in ids], [active[i][0].spec.relaxation for i in ids],
+                              nodes_per_thread=self.nodes_per_thread)
+                members = ids
+            k = min(self.chunk_steps, min(left for _, _, left in active.values()))
+            tb = time.perf_counter()
+            batch.run(k)
+            self.busy += time.perf_counter() - tb
+            finished = []
+            for i in ids:
+                rec, lat, left = active[i]
+                active[i] = (rec, lat, left - k)
+                if left - k == 0:
+                    finished.append(i)
+            if finished:
+                batch.close()
+                batch, members = None, ()
+                for i in finished:
+                    rec, lat, _ = active.pop(i)
+                    rec.result = rec.spec.result(lat)
+                    rec.finished = time.perf_counter() - t0
+                    lat.close()
+                    used -= rec.spec.footprint()
+        if batch is not None:
+            batch.close()
+        self.makespan = time.perf_counter() - t0
+        return records
+
+
+def summarize(records: Sequence[JobRecord], makespan: float) -> dict:
+    lat = np.array([r.latency for r in records])
+    work = sum(r.spec.nodes * r.spec.steps for r in records)
+    span = max(r.arrival for r in records) if records else 0.0
+    return {"jobs": len(records), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "makespan_s": makespan, "aggregate_mlups": work / makespan / 1e6,
+            "offered_mlups": work / span / 1e6 if span > 0 else None,
+            "unstable": sum(1 for r in records if not r.result.get("finite", False))}
