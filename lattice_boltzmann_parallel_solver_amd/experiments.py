@@ -206,3 +206,102 @@ def poiseuille_profile(lx: int, ly: int, delta_p: float = 0.001, omega: float = 
     return {"ux_inlet": vx[1, :], "ux_mid": vx[lx // 2, :], "area_inlet": a_in, "area_middle": a_mid,
             "ratio": a_in / a_mid, "popt": popt, "analytic": ana,
             "max_rel_error": float(np.abs(vx[lx // 2, :] - ana).max() / ana.max()), "rho": rho}
+
+
+# ----------------------------------------------------------------------------- precision study
+def precision_study(lx: int = 8192, ly: int = 2048, plate: int = 400, reynolds: float = 100.0, u_in: float = 0.1,
+                    checkpoints=(10000, 50000), strouhal_steps: int = 0, storages=("f32", "f16"),
+                    walls: bool = True, y_offset: int = 1, device: int = 0, chunk: int = 20000,
+                    progress=None) -> dict:
+    """Reduced-precision storage against the FP64 run of the same channel
+    (the FP64 device path equals the CPU oracle bit for bit).
+
+    At each checkpoint: relative L2 errors ||x - x64|| / ||x64|| of the
+    velocity and density fields (density also against rho64 - 1).  If
+    strouhal_steps > 0 every run continues to that many steps and the
+    Strouhal numbers of the probe's u_y over the second half are compared.
+    Returns a dict of results."""
+    nu = u_in * plate / reynolds
+    om = omega_from_viscosity(nu)
+    bc = von_karman_boundary_conditions(lx, ly, 1.0, u_in, plate, walls=walls, y_offset=y_offset)
+    rho0, u0 = density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), u_in)
+    total = max(max(checkpoints), strouhal_steps)
+    runs = {}
+    for st in ("f64",) + tuple(storages):
+        lat = Lattice(lx, ly, boundary=bc, storage=st, device=device)
+        lat.set_equilibrium(rho0, u0)
+        lat.set_probe(3 * lx // 4, ly // 2, total)
+        runs[st] = lat
+    out = {"lattice": [lx, ly], "plate": plate, "reynolds": reynolds, "omega": om, "checkpoints": {}}
+    done = 0
+
+    def advance(to):
+        nonlocal done
+        while done < to:
+            k = min(chunk, to - done)
+            for lat in runs.values():
+                lat.run(om, k)
+            for lat in runs.values():
+                lat.synchronize()
+            done += k
+            if progress:
+                progress(done)
+
+    for cp in sorted(checkpoints):
+        advance(cp)
+        _, r64, u64 = runs["f64"].state(want=("rho", "u"))
+        row = {}
+        for st in storages:
+            _, r, u = runs[st].state(want=("rho", "u"))
+            row[st] = {"l2_u": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
+                       "l2_rho": float(np.linalg.norm(r - r64) / np.linalg.norm(r64)),
+                       "l2_rho_dev": float(np.linalg.norm(r - r64) / max(np.linalg.norm(r64 - 1.0), 1e-300)),
+                       "max_abs_u": float(np.abs(u - u64).max())}
+        out["checkpoints"][str(cp)] = row
+    if strouhal_steps:
+        advance(strouhal_steps)
+        st_out = {}
+        for st, lat in runs.items():
+            smp = lat.probe()
+            half = len(smp) // 2
+            fft_st, _, fft_bin = strouhal_reference(smp[:, 2], plate, u_in, half)
+            st_out[st] = {"strouhal_uy_interp": strouhal_interpolated(smp[:, 2], plate, u_in, half),
+                          "strouhal_uy_zero_crossing": strouhal_zero_crossing(smp[:, 2], plate, u_in, half),
+                          "strouhal_uy_fft": fft_st, "fft_bin": fft_bin,
+                          "strouhal_speed_fft": strouhal_reference(speed(smp), plate, u_in, half)[0]}
+        for st in storages:
+            st_out[st]["rel_error_vs_f64"] = (st_out[st]["strouhal_uy_interp"] /
+                                             st_out["f64"]["strouhal_uy_interp"] - 1.0)
+        out["strouhal"] = st_out
+        out["strouhal_steps"] = strouhal_steps
+    for lat in runs.values():
+        lat.close()
+    return out
+
+
+def _main(argv=None):
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="experiment drivers on the GPU")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    p = sub.add_parser("precision", help="FP32/FP16 storage against FP64 on a von Karman channel")
+    p.add_argument("--lx", type=int, default=8192)
+    p.add_argument("--ly", type=int, default=2048)
+    p.add_argument("--plate", type=int, default=400)
+    p.add_argument("--checkpoints", type=int, nargs="+", default=[10000, 50000])
+    p.add_argument("--strouhal-steps", type=int, default=0)
+    p.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    if a.cmd == "precision":
+        res = precision_study(a.lx, a.ly, a.plate, checkpoints=a.checkpoints, strouhal_steps=a.strouhal_steps,
+                              progress=lambda n: print(f"step {n}", flush=True))
+        text = json.dumps(res, indent=1)
+        print(text)
+        if a.out:
+            with open(a.out, "w") as fh:
+                fh.write(text)
+
+
+if __name__ == "__main__":
+    _main()
