@@ -16,7 +16,9 @@ latency    : p50/p99 time-to-result of the continuous-batching server on a seede
              Poisson job stream (scaled serving mix, stated in the line).
 batching   : aggregate MLUPS of seeded uneven channels advanced by one launch per
              step versus the same channels solved one at a time.
-cpu_baseline: the oracle (numpy FP64, one core) on a few steps of the same workload.
+cpu_baseline: the oracle (numpy FP64) on a few steps of the same workload: P processes in
+             the reference's block decomposition (value), one process, and the C
+             restatement with OpenMP threads.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 """
@@ -106,24 +108,61 @@ def batching_leg(n=64, steps=1000, seed=0):
             "sequential_mlups": sequential, "speedup": batched / sequential}
 
 
-def cpu_baseline(steps=2):
-    """numpy oracle (FP64, one core) on `steps` steps of the same workload after one warm-up step."""
+def cpu_baseline(steps=2, mp_steps=6, c_steps=10):
+    """The reference's numpy path restated (the oracle, FP64) on this host's cores,
+    on a few steps of the same workload:
+
+    value          : P processes with the reference's MPI block decomposition
+                     (oracle/lbm_oracle_mp.py; P = host cores, at most 16)
+    single_process : the numpy oracle on one core
+    c_openmp       : the C restatement of the oracle with OpenMP threads
+
+    Runs before anything initialises the GPU (the multi-process leg forks)."""
+    import platform
+
     import numpy as np
 
     from oracle import lbm_oracle as O
+    from oracle import lbm_oracle_mp as MP
+    from oracle import lbm_oracle_native as NAT
 
-    rho, u = O.density_1_velocity_x_u0_velocity_y_0_initial((LX, LY), U_IN)
-    f = O.equilibrium(rho, u)
+    rho0, u0 = O.density_1_velocity_x_u0_velocity_y_0_initial((LX, LY), U_IN)
+    f0 = O.equilibrium(rho0, u0)
     bc = O.von_karman_boundary_conditions(LX, LY, 1.0, U_IN, PLATE, walls=True, y_offset=1)
-    f, rho, u = O.step(f, rho, u, omega(), bc)
+    N = LX * LY
+
+    f, rho, u = O.step(f0, rho0, u0, omega(), bc)
     t0 = time.perf_counter_ns()
     for _ in range(steps):
         f, rho, u = O.step(f, rho, u, omega(), bc)
-    dt = (time.perf_counter_ns() - t0) / 1e9
+    dt1 = (time.perf_counter_ns() - t0) / 1e9
     assert np.isfinite(rho).all()
-    return {"value": LX * LY * steps / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
-            "sample": f"{steps} steps of the {LX}x{LY} FP64 channel with the numpy oracle "
-                      f"after 1 warm-up step ({dt:.1f} s)"}
+    del f, rho, u
+
+    procs = MP.default_processes()
+    _, rho_mp, _, dtp, grid = MP.run(f0, rho0, u0, omega(), mp_steps, bc, processes=procs, warmup=1)
+    assert np.isfinite(rho_mp).all()
+    del rho_mp
+
+    NAT.run(f0, rho0, u0, omega(), 1, bc)  # build/load, first touch
+    t0 = time.perf_counter()
+    NAT.run(f0, rho0, u0, omega(), 1, bc)
+    t1 = time.perf_counter()
+    _, rho_c, _, _ = NAT.run(f0, rho0, u0, omega(), 1 + c_steps, bc)
+    t2 = time.perf_counter()
+    assert np.isfinite(rho_c).all()
+    dtc = (t2 - t1) - (t1 - t0)
+    threads = int(NAT.lib().oracle_threads())
+    host = f"{platform.processor() or platform.machine()}, numpy {np.__version__}"
+    return {"value": N * mp_steps / dtp / 1e6, "unit": "MLUPS", "cores": procs, "kind": "port",
+            "sample": f"{mp_steps} steps (after 1 warm-up) of the {LX}x{LY} FP64 channel with the numpy oracle "
+                      f"on {procs} processes in the reference's {grid[0]}x{grid[1]} block decomposition, "
+                      f"halos through shared memory ({dtp:.1f} s); host {host}",
+            "single_process": {"value": N * steps / dt1 / 1e6, "unit": "MLUPS", "cores": 1,
+                               "sample": f"{steps} steps after 1 warm-up ({dt1:.1f} s)"},
+            "c_openmp": {"value": N * c_steps / dtc / 1e6, "unit": "MLUPS", "cores": threads,
+                         "sample": f"{c_steps} steps of the C restatement of the oracle, OpenMP "
+                                   f"({dtc:.1f} s, set-up subtracted by a 1-step run)"}}
 
 
 def main():
@@ -144,6 +183,7 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the lattice is not split across devices)", file=sys.stderr)
         sys.exit(2)
+    cpu = None if (args.no_cpu_baseline or args.copy_calibration) else cpu_baseline()
 
     import torch
 
@@ -190,7 +230,6 @@ def main():
 
     latency = None if args.no_serving else serving_leg()
     batching = None if args.no_batching else batching_leg()
-    cpu = None if args.no_cpu_baseline else cpu_baseline()
 
     line = {
         "metric": METRIC,

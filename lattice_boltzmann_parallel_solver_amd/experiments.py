@@ -92,7 +92,8 @@ def strouhal_zero_crossing(trace, L: float, u: float, start: int = 0) -> float:
 # ----------------------------------------------------------------------------- drivers
 def von_karman_probe(lx: int, ly: int, plate_size: int, steps: int, nu: float = 0.04, u_in: float = 0.1,
                      rho_in: float = 1.0, storage: str = "f64", walls: bool = False, y_offset: int = 0,
-                     probe: Optional[Tuple[int, int]] = None, device: int = 0, chunk: int = 100000):
+                     probe: Optional[Tuple[int, int]] = None, device: int = 0, chunk: int = 100000,
+                     progress=None):
     """x_strouhal (experiments.py:650-720) on one GPU: uniform inflow start,
     probe (rho, ux, uy) at (3 lx//4, ly//2) after every step.
 
@@ -110,6 +111,9 @@ def von_karman_probe(lx: int, ly: int, plate_size: int, steps: int, nu: float = 
         k = min(chunk, steps - done)
         lat.run(om, k)
         done += k
+        if progress:
+            lat.synchronize()
+            progress(done)
     lat.synchronize()
     return lat.probe(), lat
 
@@ -180,6 +184,20 @@ def poiseuille_analytic(ly: int, dpdx: float, mu: float) -> np.ndarray:
     return -np.reciprocal(2 * mu) * dpdx * y * (ly - y)
 
 
+def poiseuille_startup_analytic(ly: int, accel: float, nu: float, t: float, terms: int = 4001) -> np.ndarray:
+    """Start-up plane Poiseuille flow from rest under a uniform driving
+    acceleration G = -dp/dx / rho between walls at y = -1/2 and y = ly - 1/2:
+    u = G/(2 nu) eta (h - eta) - sum over odd n of
+    4 G h^2/(nu pi^3 n^3) sin(n pi eta/h) exp(-nu (n pi/h)^2 t), eta = y + 1/2."""
+    h = float(ly)
+    eta = np.arange(ly) + 0.5
+    u = accel / (2 * nu) * eta * (h - eta)
+    for n in range(1, terms + 1, 2):
+        u -= (4 * accel * h * h / (nu * np.pi ** 3 * n ** 3)) * np.sin(n * np.pi * eta / h) * \
+            np.exp(-nu * (n * np.pi / h) ** 2 * t)
+    return u
+
+
 def poiseuille_profile(lx: int, ly: int, delta_p: float = 0.001, omega: float = 1.5, steps: int = 40000,
                        storage: str = "f64", device: int = 0):
     """plot_poiseuille_flow_vel_vectors (experiments.py:377-504): pressure
@@ -203,9 +221,14 @@ def poiseuille_profile(lx: int, ly: int, delta_p: float = 0.001, omega: float = 
     popt, _ = curve_fit(lambda y, a, b, c: a * y ** 2 + b * y + c, np.arange(ly), vx[lx // 2, :])
     mu = viscosity_from_omega(omega) * np.mean(rho[lx // 2, :])
     ana = poiseuille_analytic(ly, (p_out - p_in) / lx, mu)
+    rho_mid = float(np.mean(rho[lx // 2, :]))
+    startup = poiseuille_startup_analytic(ly, -(p_out - p_in) / lx / rho_mid, viscosity_from_omega(omega), steps)
     return {"ux_inlet": vx[1, :], "ux_mid": vx[lx // 2, :], "area_inlet": a_in, "area_middle": a_mid,
             "ratio": a_in / a_mid, "popt": popt, "analytic": ana,
-            "max_rel_error": float(np.abs(vx[lx // 2, :] - ana).max() / ana.max()), "rho": rho}
+            "max_rel_error": float(np.abs(vx[lx // 2, :] - ana).max() / ana.max()),
+            "analytic_startup": startup,
+            "max_rel_error_startup": float(np.abs(vx[lx // 2, :] - startup).max() / np.abs(startup).max()),
+            "rho": rho}
 
 
 # ----------------------------------------------------------------------------- precision study
@@ -279,6 +302,48 @@ def precision_study(lx: int = 8192, ly: int = 2048, plate: int = 400, reynolds: 
     return out
 
 
+def channel_checks(lx: int = 4200, ly: int = 1800, plate: int = 400, steps: int = 50000, box: int = 2048,
+                   box_steps: int = 50000, u_in: float = 0.1, reynolds: float = 100.0, device: int = 0,
+                   progress=None) -> dict:
+    """FP64 channel workload and the two box checks of the reference's
+    experiments at a large size:
+
+    * von Karman channel lx x ly, plate `plate` at lx//4 (+1 y offset), rigid
+      walls, Re from nu = u_in*plate/Re; probe u_y at (3 lx//4, ly//2);
+      Strouhal by the reference's FFT peak and by the sub-bin estimators over
+      the second half of the trace.
+    * Couette box x box (lid u = 0.05, omega 1) against the start-up Fourier
+      series, Poiseuille box x box (delta p = 1e-3, omega 1.5) against the
+      start-up series and the flux balance between x = 1 and x = box//2."""
+    out = {}
+    nu = u_in * plate / reynolds
+    smp, lat = von_karman_probe(lx, ly, plate, steps, nu=nu, u_in=u_in, walls=True, y_offset=1, device=device,
+                                chunk=10000, progress=(lambda n: progress("channel", n)) if progress else None)
+    m, jx, jy = lat.moments_sum()
+    lat.close()
+    half = steps // 2
+    st_fft, f_peak, bin_w = strouhal_reference(smp[:, 2], plate, u_in, half)
+    out["channel"] = {"lattice": [lx, ly], "plate": plate, "steps": steps, "omega": omega_from_viscosity(nu),
+                      "probe": [3 * lx // 4, ly // 2], "strouhal_uy_fft": st_fft, "fft_bin": bin_w,
+                      "strouhal_uy_interp": strouhal_interpolated(smp[:, 2], plate, u_in, half),
+                      "strouhal_uy_zero_crossing": strouhal_zero_crossing(smp[:, 2], plate, u_in, half),
+                      "strouhal_speed_fft": strouhal_reference(speed(smp), plate, u_in, half)[0],
+                      "mass": m, "finite": bool(np.isfinite(smp).all())}
+    if progress:
+        progress("couette", 0)
+    c = couette_profile(box, box, U=0.05, omega=1.0, steps=box_steps, device=device)
+    out["couette"] = {"lattice": [box, box], "steps": box_steps, "max_abs_error": c["max_abs_error"],
+                      "max_rel_error": c["max_abs_error"] / 0.05}
+    if progress:
+        progress("poiseuille", 0)
+    p = poiseuille_profile(box, box, delta_p=1e-3, omega=1.5, steps=box_steps, device=device)
+    out["poiseuille"] = {"lattice": [box, box], "steps": box_steps, "flux_ratio_inlet_middle": p["ratio"],
+                         "max_rel_error_startup": p["max_rel_error_startup"],
+                         "u_centre": float(p["ux_mid"][box // 2]),
+                         "u_centre_startup": float(p["analytic_startup"][box // 2])}
+    return out
+
+
 def _main(argv=None):
     import argparse
     import json
@@ -292,7 +357,23 @@ def _main(argv=None):
     p.add_argument("--checkpoints", type=int, nargs="+", default=[10000, 50000])
     p.add_argument("--strouhal-steps", type=int, default=0)
     p.add_argument("--out", default=None)
+    c = sub.add_parser("channel-checks", help="FP64 von Karman Strouhal plus Couette and Poiseuille profiles")
+    c.add_argument("--lx", type=int, default=4200)
+    c.add_argument("--ly", type=int, default=1800)
+    c.add_argument("--plate", type=int, default=400)
+    c.add_argument("--steps", type=int, default=50000)
+    c.add_argument("--box", type=int, default=2048)
+    c.add_argument("--box-steps", type=int, default=50000)
+    c.add_argument("--out", default=None)
     a = ap.parse_args(argv)
+    if a.cmd == "channel-checks":
+        res = channel_checks(a.lx, a.ly, a.plate, a.steps, a.box, a.box_steps,
+                             progress=lambda tag, n: print(f"{tag} step {n}", flush=True))
+        text = json.dumps(res, indent=1)
+        print(text)
+        if a.out:
+            with open(a.out, "w") as fh:
+                fh.write(text)
     if a.cmd == "precision":
         res = precision_study(a.lx, a.ly, a.plate, checkpoints=a.checkpoints, strouhal_steps=a.strouhal_steps,
                               progress=lambda n: print(f"step {n}", flush=True))

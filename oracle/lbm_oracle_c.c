@@ -8,13 +8,15 @@
  * It performs the same IEEE operations in the same order as oracle/lbm_oracle.py
  * (which restates src/lattice_boltzmann_method.py:93-228 and
  * src/boundary_conditions.py:78-348 of the reference), so the two are
- * bit-identical; tests/test_oracle_c.py checks that.  Build with
+ * bit-identical; tests/test_oracle.py
+ * (test_c_oracle_equals_numpy_oracle_on_all_composites) checks that.  Build with
  * -ffp-contract=off and without -ffast-math: a fused multiply-add would change
  * the rounding.
  *
  * Layout: AoS f[x][y][i], C order, lx*ly*9 doubles, as in the reference.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -180,3 +182,6 @@ int oracle_run(int lx, int ly, double *f, double *rho, double *u, double omega, 
 void oracle_norm_fma(const double *ux, const double *uy, double *out, long n) {
     for (long i = 0; i < n; i++) out[i] = sqrt(fma(uy[i], uy[i], ux[i] * ux[i]));
 }
+
+/* OpenMP threads a parallel region of oracle_run uses. */
+int oracle_threads(void) { return omp_get_max_threads(); }

@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY: the checker used by tests/ for long runs and by
 bench.py's cpu_baseline leg.  It runs the same arithmetic as
-oracle/lbm_oracle.py; tests/test_oracle_c.py checks the two bit for bit.
+oracle/lbm_oracle.py; tests/test_oracle.py checks the two bit for bit.
 """
 
 from __future__ import annotations
@@ -44,8 +44,7 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB):
-            build()
+        build()
         L = ctypes.CDLL(_LIB)
         P = ctypes.POINTER(ctypes.c_double)
         L.oracle_run.argtypes = [ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_double, ctypes.c_long,
@@ -54,6 +53,7 @@ def lib():
         L.oracle_set_square_pow.argtypes = [ctypes.c_int]
         L.oracle_equilibrium.argtypes = [P, P, P, ctypes.c_long]
         L.oracle_norm_fma.argtypes = [P, P, P, ctypes.c_long]
+        L.oracle_threads.restype = ctypes.c_int
         _lib = L
     return _lib
 

@@ -66,3 +66,15 @@ def test_poiseuille_analytic_is_parabolic_and_symmetric():
 def test_viscosity_omega_round_trip():
     for nu in (0.01, 0.04, 1 / 6, 0.4):
         assert E.viscosity_from_omega(E.omega_from_viscosity(nu)) == pytest.approx(nu, rel=1e-13)
+
+
+def test_poiseuille_startup_series_limits():
+    """At t = 0 the start-up series cancels the steady parabola; for t much
+    longer than h^2/nu it is the steady profile; early on the core accelerates
+    uniformly (u = G t) away from the walls."""
+    ly, g, nu = 96, 2e-7, 1 / 6
+    steady = E.poiseuille_analytic(ly, -g, nu)
+    assert np.abs(E.poiseuille_startup_analytic(ly, g, nu, 0.0)).max() < 1e-9 * steady.max()
+    assert np.allclose(E.poiseuille_startup_analytic(ly, g, nu, 50 * ly * ly / nu), steady, rtol=0, atol=1e-15)
+    t = 20.0
+    assert abs(E.poiseuille_startup_analytic(ly, g, nu, t)[ly // 2] - g * t) < 1e-6 * g * t
