@@ -5,12 +5,17 @@ Workload (the precision configuration of the reference's channel scaled up):
 offset, inlet equilibrium u = (0.1, 0), rho = 1, outlet extrapolation, rigid
 top and bottom walls, nu = 0.1*400/100 (Re = 100) so omega = 1/1.7.
 
-value      : MLUPS of the FP32-deviation step (inputs resident in HBM).
-roofline   : HBM bound; achieved = algorithmic bytes per launch (18 population
-             accesses x 4 B per node) / average launch time over the timed region
-             (HIP events on the stream the kernels run on); traffic = HBM bytes per
-             launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated
+value      : MLUPS of the FP32-deviation step (inputs resident in HBM), two steps
+             per launch (the row-sweep kernel keeps the intermediate step on chip).
+roofline   : HBM bound; achieved = algorithmic bytes per launch (every population
+             read and written once: 18 accesses x 4 B per node) / average launch
+             time over the timed region (HIP events on the stream the kernels run
+             on); traffic = HBM bytes per launch measured with rocprofv3
+             FETCH_SIZE/WRITE_SIZE, copy-calibrated
              (profiles/fused_step_8192x2048_hbm_traffic.json).
+             single_step_equivalent_gbs: the bandwidth a one-step-per-launch kernel
+             would need for the same MLUPS (72 B per node step).
+single_step: the same workload with one step per launch (k_fused).
 fp64, f16  : the same workload with FP64 populations / FP16 deviation storage.
 latency    : p50/p99 time-to-result of the continuous-batching server on a seeded
              Poisson job stream (scaled serving mix, stated in the line).
@@ -41,11 +46,14 @@ BYTES_PER_NODE = {"f64": 144, "f32": 72, "f16": 36}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fused_step_8192x2048_hbm_traffic.json")
 
 
-def measured_traffic(storage):
-    """HBM bytes per fused-step launch from the committed rocprofv3 counter runs."""
+def measured_traffic(storage, steps_per_launch=1):
+    """HBM bytes per launch of the step kernel from the committed rocprofv3
+    counter runs (keys "f32", "f64", "f16" for one step per launch,
+    "f32_sweep2", ... for the two-step kernel)."""
+    key = storage if steps_per_launch == 1 else f"{storage}_sweep2"
     try:
         with open(TRAFFIC_JSON) as fh:
-            return float(json.load(fh)[storage]["traffic_bytes"])
+            return float(json.load(fh)[key]["traffic_bytes"])
     except (OSError, KeyError, ValueError):
         return None
 
@@ -55,7 +63,9 @@ def omega():
     return 1.0 / (3.0 * nu + 0.5)
 
 
-def build_lattice(storage, torch):
+def build_lattice(storage, torch, temporal=1):
+    """The bench channel, advanced `temporal` steps per launch where the
+    two-step kernel applies (falls back to one)."""
     from lattice_boltzmann_parallel_solver_amd import Lattice
     from lattice_boltzmann_parallel_solver_amd.boundary_utils import von_karman_boundary_conditions
 
@@ -66,6 +76,15 @@ def build_lattice(storage, torch):
     u[..., 0] = U_IN
     lat.set_equilibrium(rho, u)
     del rho, u
+    lat.steps_per_launch = 1
+    if temporal == 2:
+        from lattice_boltzmann_parallel_solver_amd._lib import LbmError
+
+        try:
+            lat.set_temporal(2)
+            lat.steps_per_launch = 2
+        except LbmError:
+            pass
     return lat
 
 
@@ -173,6 +192,8 @@ def main():
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "f64"])
     ap.add_argument("--fp64-steps", type=int, default=200)
     ap.add_argument("--f16-steps", type=int, default=200)
+    ap.add_argument("--temporal", type=int, default=2, choices=[1, 2],
+                    help="steps per kernel launch (2: row-sweep kernel where it applies)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
@@ -200,33 +221,39 @@ def main():
         return
     N = LX * LY
 
-    lat = build_lattice(args.storage, torch)
-    ms = time_steps(lat, args.steps, args.warmup, torch)
+    steps = args.steps + args.steps % 2  # whole launches of the two-step kernel
+    lat = build_lattice(args.storage, torch, args.temporal)
+    spl = lat.steps_per_launch
+    ms = time_steps(lat, steps, args.warmup, torch)
     m, jx, _ = lat.moments_sum()
     assert m == m and abs(m / N - 1.0) < 0.1, "unstable run"
     lat.close()
     del lat
     torch.cuda.empty_cache()
     sec = ms / 1e3
-    mlups = N * args.steps / sec / 1e6
-    launch_s = sec / args.steps
+    mlups = N * steps / sec / 1e6
+    launch_s = sec / (steps / spl)
+    # one launch reads and writes every population once, whatever its step count
     bytes_launch = BYTES_PER_NODE[args.storage] * N
     achieved = bytes_launch / launch_s / 1e9
 
-    def leg(storage, steps):
-        if steps <= 0 or storage == args.storage:
+    def leg(storage, steps, temporal):
+        if steps <= 0:
             return None
-        lat = build_lattice(storage, torch)
-        ms_ = time_steps(lat, steps, max(5, args.warmup // 4), torch)
+        steps += steps % 2
+        lat = build_lattice(storage, torch, temporal)
+        spl_ = lat.steps_per_launch
+        ms_ = time_steps(lat, steps, max(6, args.warmup // 4), torch)
         lat.close()
         torch.cuda.empty_cache()
-        ach = BYTES_PER_NODE[storage] * N / (ms_ / 1e3 / steps) / 1e9
+        ach = BYTES_PER_NODE[storage] * N / (ms_ / 1e3 / (steps / spl_)) / 1e9
         return {"mlups": N * steps / (ms_ / 1e3) / 1e6, "ms_per_step": ms_ / steps, "steps": steps,
-                "roofline_achieved_gbs": ach, "roofline_frac": ach / HBM_PEAK_GBS,
-                "bytes_per_node": BYTES_PER_NODE[storage], "traffic": measured_traffic(storage)}
+                "steps_per_launch": spl_, "roofline_achieved_gbs": ach, "roofline_frac": ach / HBM_PEAK_GBS,
+                "bytes_per_node_step": BYTES_PER_NODE[storage] // spl_, "traffic": measured_traffic(storage, spl_)}
 
-    fp64 = leg("f64", args.fp64_steps)
-    f16 = leg("f16", args.f16_steps)
+    single = leg(args.storage, args.fp64_steps, 1) if spl == 2 else None
+    fp64 = leg("f64", args.fp64_steps, args.temporal) if args.storage != "f64" else None
+    f16 = leg("f16", args.f16_steps, args.temporal) if args.storage != "f16" else None
 
     latency = None if args.no_serving else serving_leg()
     batching = None if args.no_batching else batching_leg()
@@ -236,9 +263,9 @@ def main():
         "value": mlups,
         "unit": "MLUPS",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": ms / args.steps,
+        "ms_per_step": ms / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -246,11 +273,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"von Karman channel {LX}x{LY}, plate {PLATE} at x={LX // 4} (+1 y offset), "
                                f"rigid walls, inlet u={U_IN}, Re={RE:g}, {args.storage} population storage",
-                   "lx": LX, "ly": LY, "storage": args.storage, "omega": omega(),
+                   "lx": LX, "ly": LY, "storage": args.storage, "omega": omega(), "steps_per_launch": spl,
                    "parallelism": "single GPU (whole lattice resident)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.storage),
-                     "bytes_per_node": BYTES_PER_NODE[args.storage]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.storage, spl),
+                     "kernel": "k_sweep2 (two steps per launch)" if spl == 2 else "k_fused (one step per launch)",
+                     "bytes_per_node_launch": BYTES_PER_NODE[args.storage],
+                     "bytes_per_node_step": BYTES_PER_NODE[args.storage] // spl,
+                     "single_step_equivalent_gbs": mlups * 1e6 * BYTES_PER_NODE[args.storage] / 1e9},
+        "single_step": single,
         "fp64": fp64,
         "f16": f16,
         "latency": latency,

@@ -140,6 +140,15 @@ int lbm_steps_done(const lbm_ctx *ctx, int64_t *steps);
  * block order (0/1), non-temporal stores (0/1).  Default {1, 1, 0}, the
  * fastest measured on MI355X for every storage. */
 int lbm_set_launch(lbm_ctx *ctx, int cells_per_thread, int xcd_remap, int nt_stores);
+/* Temporal blocking of lbm_run: steps_per_launch = 2 advances two steps per
+ * kernel launch (a row sweep that keeps the intermediate state on chip and
+ * reads and writes the populations once per two steps; results are identical
+ * to single steps), 1 = one step per launch (default).  rows_per_block: rows
+ * each block of the two-step kernel sweeps (0 = automatic).  Two steps need
+ * no pressure fix in the boundary program, nx >= 3 and ny <= 2048 (FP32,
+ * FP16) or 1024 (FP64); otherwise LBM_ERR_ARG.  No counterpart in the
+ * reference (a launch setting of this build). */
+int lbm_set_temporal(lbm_ctx *ctx, int steps_per_launch, int rows_per_block);
 
 /* ---- batches: many lattices advanced by one launch per step ------------- */
 /* A batch groups contexts (same device, same storage) so that one kernel

@@ -77,6 +77,7 @@ hipError_t upload_bc(const lbm_bc_prog &p, BcUpload &up) {
     up.view.codes = reinterpret_cast<const uint16_t *>(mem + o_codes);
     up.view.chunk = reinterpret_cast<const uint8_t *>(mem + o_chunk);
     up.view.classes = reinterpret_cast<const ClassEntry *>(mem + o_cls);
+    up.view.nclasses = (uint32_t)p.classes.size();
     up.view.special = reinterpret_cast<const uint32_t *>(mem + o_spec);
     up.view.nspecial = (int64_t)p.special.size();
     up.view.rho_in = p.rho_in;
@@ -263,6 +264,27 @@ int lbm_set_launch(lbm_ctx *c, int v, int xcd, int nt) {
     return LBM_OK;
 }
 
+int lbm_set_temporal(lbm_ctx *c, int steps_per_launch, int rows_per_block) {
+    int rc = check_ctx(c, "lbm_set_temporal");
+    if (rc) return rc;
+    if (steps_per_launch != 1 && steps_per_launch != 2) {
+        set_error("lbm_set_temporal: steps_per_launch must be 1 or 2");
+        return LBM_ERR_ARG;
+    }
+    if (rows_per_block < 0) {
+        set_error("lbm_set_temporal: rows_per_block must be >= 0 (0 = automatic)");
+        return LBM_ERR_ARG;
+    }
+    if (steps_per_launch == 2 && !sweep_supported(c->storage, c->g, c->bc.view)) {
+        set_error("lbm_set_temporal: two steps per launch need no pressure fix, at most 256 boundary classes, "
+                  "nx >= 3 and ny <= 2048 (FP32, FP16) or 1024 (FP64)");
+        return LBM_ERR_ARG;
+    }
+    c->cfg.steps_per_launch = steps_per_launch;
+    c->cfg.sweep_rows = rows_per_block;
+    return LBM_OK;
+}
+
 // Materialise f_n, rho_n, u_n of a running context into (device) outputs.
 static int finalize_into(lbm_ctx *c, double *f, double *rho, double *u) {
     LBM_HIP_CHECK(launch_finalize(c->storage, c->g, c->st, c->cur, c->bc.view, f, rho, u, nullptr, c->stream,
@@ -357,6 +379,17 @@ int lbm_run(lbm_ctx *c, double omega, int64_t nsteps) {
         left--;
     }
     const bool probing = c->probe.cell >= 0;
+    if (c->cfg.steps_per_launch == 2 && sweep_supported(c->storage, c->g, c->bc.view)) {
+        for (; left >= 2; left -= 2) {
+            // materialises f_{steps} and f_{steps+1}
+            const int64_t slot = probing ? c->steps - c->probe_base - 1 : -1;
+            LBM_HIP_CHECK(launch_sweep2(c->storage, c->g, c->st, c->cur, c->bc.view, omega, c->probe, slot,
+                                        probing ? slot + 1 : -1, c->cfg, c->stream),
+                          "two-step sweep kernel");
+            c->cur ^= 1;
+            c->steps += 2;
+        }
+    }
     for (; left > 0; left--) {
         // this launch materialises f_{steps} in registers: record it if probed
         const int64_t slot = probing ? c->steps - c->probe_base - 1 : -1;
@@ -830,6 +863,7 @@ static int prog_on_device(const lbm_bc_prog *prog, BcView &view, const char *fn)
     view.codes = p->dev_codes;
     view.chunk = p->dev_chunk;
     view.classes = p->dev_classes;
+    view.nclasses = (uint32_t)p->classes.size();
     view.special = p->dev_special;
     view.nspecial = (int64_t)p->special.size();
     view.rho_in = p->rho_in;

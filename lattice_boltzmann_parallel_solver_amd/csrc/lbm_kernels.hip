@@ -68,13 +68,22 @@ struct Io<float> {
     }
 };
 
+// FP32 -> FP16 with round-to-nearest-even, always from the FP32-rounded value.
+// The empty asm hides the value's origin from the instruction combiner, which
+// would otherwise fold a preceding fmaf into v_fma_mix (one rounding straight
+// to FP16 instead of FP32 then FP16) in some kernels and not in others.
+__device__ __forceinline__ __half f32_to_f16(float v) {
+    asm("" : "+v"(v));
+    return __float2half_rn(v);
+}
+
 template <>
 struct Io<__half> {
     using C = float;
     static constexpr bool kDev = true;
     __device__ static C ld(const __half *p) { return __half2float(*p); }
     __device__ static void st(__half *p, C v, bool nt) {
-        unsigned short bits = __half_as_ushort(__float2half_rn(v));
+        unsigned short bits = __half_as_ushort(f32_to_f16(v));
         if (nt) __builtin_nontemporal_store(bits, reinterpret_cast<unsigned short *>(p));
         else *reinterpret_cast<unsigned short *>(p) = bits;
     }
@@ -128,6 +137,31 @@ __device__ __forceinline__ void pull(const S *__restrict__ src, const Geom &g, u
 #pragma unroll
     for (int i = 0; i < 9; i++)
         out[i] = Io<S>::ld(src + pop(g, src_row(g, x, kCX[i]), i, src_col(g, y, kCY[i])));
+}
+
+// Element `off` of a state buffer through a 32-bit byte offset (lets the
+// compiler use the uniform-base + 32-bit-offset addressing form).  Valid while
+// 9 * N * sizeof(S) < 2^32.
+template <class S>
+__device__ __forceinline__ const S *at32(const S *base, uint32_t off) {
+    return reinterpret_cast<const S *>(reinterpret_cast<const char *>(base) + (uint32_t)(off * (uint32_t)sizeof(S)));
+}
+template <class S>
+__device__ __forceinline__ S *at32(S *base, uint32_t off) {
+    return reinterpret_cast<S *>(reinterpret_cast<char *>(base) + (uint32_t)(off * (uint32_t)sizeof(S)));
+}
+
+// pull() with 32-bit offsets: the three source rows are given.
+template <class S>
+__device__ __forceinline__ void pull32(const S *__restrict__ src, uint32_t ly, uint32_t xm, uint32_t x,
+                                       uint32_t xp, uint32_t y, uint32_t ym, uint32_t yp,
+                                       typename Io<S>::C out[9]) {
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const uint32_t r = kCX[i] == 1 ? xm : (kCX[i] == -1 ? xp : x);
+        const uint32_t c = kCY[i] == 1 ? ym : (kCY[i] == -1 ? yp : y);
+        out[i] = Io<S>::ld(at32(src, (r * 9 + i) * ly + c));
+    }
 }
 
 // Apply the node's boundary class to the pulled populations; returns the
@@ -218,6 +252,9 @@ __device__ __forceinline__ void collide_node(const typename Io<S>::C v[9], doubl
         o.ux = ux;
         o.uy = uy;
     } else {
+        // Contraction off and the fused multiply-adds written out, so that every
+        // kernel rounds the reduced-precision collision the same way.
+#pragma clang fp contract(off)
         const float om = (float)omega;
         const float drho = (((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]))) + v[8];
         const float rho = 1.0f + drho;
@@ -225,14 +262,14 @@ __device__ __forceinline__ void collide_node(const typename Io<S>::C v[9], doubl
         const float jy = ((v[2] - v[4]) + (v[5] - v[7])) + (v[6] - v[8]);
         const float inv = rho != 0.0f ? 1.0f / rho : 0.0f;
         const float ux = jx * inv, uy = jy * inv;
-        const float base = -1.5f * (ux * ux + uy * uy);
+        const float base = -1.5f * __builtin_fmaf(ux, ux, uy * uy);
         const float cu[9] = {0.0f, ux, uy, -ux, -uy, ux + uy, uy - ux, -(ux + uy), ux - uy};
 #pragma unroll
         for (int i = 0; i < 9; i++) {
             // f_eq,i - w_i = w_i * (drho + rho*(3 cu + 4.5 cu^2 - 1.5 u^2))
-            const float t = cu[i] * (3.0f + 4.5f * cu[i]) + base;
-            o.e[i] = (float)kWd[i] * (drho + rho * t);
-            o.s[i] = v[i] + om * (o.e[i] - v[i]);
+            const float t = __builtin_fmaf(cu[i], __builtin_fmaf(4.5f, cu[i], 3.0f), base);
+            o.e[i] = (float)kWd[i] * __builtin_fmaf(rho, t, drho);
+            o.s[i] = __builtin_fmaf(om, o.e[i] - v[i], v[i]);
         }
         o.rho = 1.0 + (double)drho;
         o.jx = jx;
@@ -248,12 +285,14 @@ __device__ __forceinline__ void feq_rep(double rho_b, const NodeOut<S> &o, typen
     if constexpr (!Io<S>::kDev) {
         feq_f64(rho_b, o.ux, o.uy, e);
     } else {
+#pragma clang fp contract(off)
         const float ux = o.ux, uy = o.uy;
         const float r = (float)rho_b, dr = (float)(rho_b - 1.0);
-        const float base = -1.5f * (ux * ux + uy * uy);
+        const float base = -1.5f * __builtin_fmaf(ux, ux, uy * uy);
         const float cu[9] = {0.0f, ux, uy, -ux, -uy, ux + uy, uy - ux, -(ux + uy), ux - uy};
 #pragma unroll
-        for (int i = 0; i < 9; i++) e[i] = (float)kWd[i] * (dr + r * (cu[i] * (3.0f + 4.5f * cu[i]) + base));
+        for (int i = 0; i < 9; i++)
+            e[i] = (float)kWd[i] * __builtin_fmaf(r, __builtin_fmaf(cu[i], __builtin_fmaf(4.5f, cu[i], 3.0f), base), dr);
     }
 }
 
@@ -441,7 +480,7 @@ struct Pack<__half> {
     __device__ static float half_bits(unsigned b) { return __half2float(__ushort_as_half((unsigned short)b)); }
     __device__ static float get(const raw4 &r, int k) { return half_bits((r[k >> 1] >> (16 * (k & 1))) & 0xffffu); }
     __device__ static void set(raw4 &r, int k, float v) {
-        const unsigned b = __half_as_ushort(__float2half_rn(v));
+        const unsigned b = __half_as_ushort(f32_to_f16(v));
         const int sh = 16 * (k & 1);
         r[k >> 1] = (r[k >> 1] & ~(0xffffu << sh)) | (b << sh);
     }
@@ -611,6 +650,337 @@ __global__ __launch_bounds__(kBlock) void k_fused_batch(const BatchJob *__restri
 }
 
 __global__ void k_next_step(int64_t *step) { *step += 1; }
+
+// ---------------------------------------------------------------------------
+// two steps per launch: temporal blocking over whole rows
+//
+// A block owns whole rows [xa, xb) of the output and all ly nodes of each row
+// (thread t handles columns t, t + T, ...).  It sweeps x upwards; iteration
+// it = 0 .. R+1 (R = xb - xa)
+//   1. runs step 1 for row x = xa - 1 + it (periodic): pull from the input
+//      buffer, boundary class, collision; the relaxed populations s1 stay on
+//      chip, rounded to the storage type exactly as a store would round them;
+//   2. publishes the six y-moving populations of s1 in LDS, and after one
+//      barrier reads them back shifted by one node in y (periodic): together
+//      with the three populations that do not move in y this is row x of s1
+//      streamed in y;
+//   3. from it = 2 on, runs step 2 for row x - 1: directions with cx = +1
+//      come from the y-streamed row x - 2, cx = 0 from row x - 1, cx = -1
+//      from row x (the first two are carried in registers), boundary class
+//      (reflections read the node's own s1, the outlet reads the step-1
+//      outlet values kept in LDS), collision, store to the output buffer.
+// Every value passes through the same arithmetic as in two single steps, so
+// the result equals two k_fused launches bit for bit.  The input is read once
+// (plus the two rows on each side that neighbouring blocks also read) and the
+// output written once per two steps.  The step-1 loads of the next row are
+// issued before the current row is processed.  Not used with the pressure fix,
+// whose producers write rows owned by other blocks.
+// ---------------------------------------------------------------------------
+constexpr int kSweepThreads = 1024;
+// y-moving directions: the first three arrive from y - 1, the last three from y + 1.
+constexpr int kYM[6] = {2, 5, 6, 4, 7, 8};
+constexpr int kYSlot[9] = {-1, -1, 0, -1, 3, 1, 2, 4, 5};
+
+template <class S>
+struct SweepParams {
+    const S *src;
+    S *dst;
+    const typename Io<S>::C *side_prev;
+    typename Io<S>::C *side_next;
+    Geom g;
+    BcView bc;
+    double omega;
+    int64_t probe_cell;
+    double *probe1;  // sample of step 1 (3 doubles) or null
+    double *probe2;  // sample of step 2
+    uint32_t rows;   // output rows per block
+    uint32_t nchunks;
+    int nt;
+};
+
+template <class S>
+__device__ __forceinline__ typename Io<S>::C round_to_storage(typename Io<S>::C v) {
+    if constexpr (sizeof(S) == 2) return __half2float(f32_to_f16(v));
+    else return v;
+}
+
+template <class S>
+__device__ __forceinline__ void lds_put(S *p, typename Io<S>::C v) {
+    if constexpr (sizeof(S) == 2) *p = f32_to_f16(v);
+    else *p = v;
+}
+
+template <class S>
+__device__ __forceinline__ typename Io<S>::C lds_get(const S *p) {
+    if constexpr (sizeof(S) == 2) return __half2float(*p);
+    else return *p;
+}
+
+constexpr uint32_t kSweepMaxClasses = 256;
+constexpr uint32_t kSweepMaxRows = 256;
+
+__host__ __device__ constexpr size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+// Dynamic LDS of the sweep kernel: the y-exchange rows (two buffers of six
+// planes), the step-1 outlet values, the class table and one "special" byte
+// per (row, wave column) of the block's rows.
+struct SweepLds {
+    size_t side, cls, flag, total;
+};
+
+template <class S>
+__host__ __device__ inline SweepLds sweep_lds_layout(uint32_t ly, uint32_t nclasses, uint32_t rows) {
+    using C = typename Io<S>::C;
+    SweepLds L;
+    L.side = align16(12 * (size_t)ly * sizeof(S));
+    L.cls = align16(L.side + 3 * (size_t)ly * sizeof(C));
+    L.flag = align16(L.cls + (size_t)nclasses * sizeof(ClassEntry));
+    L.total = L.flag + (size_t)(rows + 2) * ((ly + 63) / 64);
+    return L;
+}
+
+// pull32() for a node with a boundary class: directions the class reflects
+// load the node's own opposite population (the reflection's source) in place
+// of the upstream one, so that no load is needed later.
+template <class S>
+__device__ __forceinline__ void pull32_bc(const S *__restrict__ src, uint32_t ly, uint32_t xm, uint32_t x,
+                                          uint32_t xp, uint32_t y, uint32_t ym, uint32_t yp, uint32_t w,
+                                          typename Io<S>::C out[9]) {
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const uint32_t r = kCX[i] == 1 ? xm : (kCX[i] == -1 ? xp : x);
+        const uint32_t c = kCY[i] == 1 ? ym : (kCY[i] == -1 ? yp : y);
+        const bool refl = ((w >> (2 * i)) & 3u) == kSrcReflect;
+        const uint32_t off = refl ? (x * 9 + kOpp[i]) * ly + y : (r * 9 + i) * ly + c;
+        out[i] = Io<S>::ld(at32(src, off));
+    }
+}
+
+template <class S, int V, bool kProbe>
+__global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
+    using C = typename Io<S>::C;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Geom &g = p.g;
+    const uint32_t ly = g.ly, lx = g.lx;
+    const uint32_t T = blockDim.x;
+    const uint32_t xa = blockIdx.x * p.rows;
+    const uint32_t R = min(p.rows, lx - xa);
+    const uint32_t nw = (ly + 63) / 64;
+    const bool has_bc = p.bc.chunk != nullptr;
+    const SweepLds L = sweep_lds_layout<S>(ly, has_bc ? p.bc.nclasses : 0, p.rows);
+    S *ex = reinterpret_cast<S *>(smem);
+    C *side = reinterpret_cast<C *>(smem + L.side);
+    ClassEntry *cls = reinterpret_cast<ClassEntry *>(smem + L.cls);
+    uint8_t *wflag = smem + L.flag;
+    // row of iteration it (step-1 row xa - 1 + it, periodic)
+    auto row_at = [&](uint32_t it) {
+        uint32_t r = xa + it + lx - 1;
+        while (r >= lx) r -= lx;
+        return r;
+    };
+    if (has_bc) {
+        const uint32_t words = p.bc.nclasses * (uint32_t)(sizeof(ClassEntry) / 4);
+        for (uint32_t i = threadIdx.x; i < words; i += T)
+            reinterpret_cast<uint32_t *>(cls)[i] = reinterpret_cast<const uint32_t *>(p.bc.classes)[i];
+        for (uint32_t i = threadIdx.x; i < (R + 2) * nw; i += T) {
+            const uint32_t rr = i / nw, j = i - rr * nw;
+            const uint32_t n0 = row_at(rr) * ly + 64 * j;
+            const uint32_t n1 = min(n0 + 63, row_at(rr) * ly + ly - 1);
+            wflag[i] = p.bc.chunk[n0 >> 6] | p.bc.chunk[n1 >> 6];
+        }
+        __syncthreads();
+    }
+    uint32_t ys[V], ym[V], yp[V], jw[V];
+    bool live[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+        ys[k] = threadIdx.x + k * T;
+        live[k] = ys[k] < ly;
+        ym[k] = ys[k] == 0 ? ly - 1 : ys[k] - 1;
+        yp[k] = ys[k] + 1 >= ly ? 0 : ys[k] + 1;
+        jw[k] = (threadIdx.x >> 6) + k * (T >> 6);
+    }
+    // the wave holding column k has boundary nodes in row iteration rr (wave-uniform)
+    auto special = [&](uint32_t rr, int k) -> bool {
+        if (!has_bc || jw[k] >= nw) return false;
+        return __builtin_amdgcn_readfirstlane(wflag[rr * nw + jw[k]]) != 0;
+    };
+    // wave-uniform branch only, and an in-bounds address for every lane: a
+    // per-lane branch would make the compiler copy the loaded value at the
+    // join, i.e. wait for it (and for the prefetch issued before it) at once
+    auto load_code = [&](uint32_t rr, int k) -> uint32_t {
+        if (!special(rr, k)) return 0;
+        return p.bc.codes[row_at(rr) * ly + (live[k] ? ys[k] : ly - 1)];
+    };
+    auto pull_row = [&](uint32_t rr, const uint32_t code[V], C out[V][9]) {
+        const uint32_t xr = row_at(rr), xm = xr == 0 ? lx - 1 : xr - 1, xp = xr + 1 == lx ? 0 : xr + 1;
+#pragma unroll
+        for (int k = 0; k < V; k++) {
+            if (!live[k]) continue;
+            if (special(rr, k)) {
+                const uint32_t w = code[k] ? cls[code[k]].src : 0u;
+                pull32_bc<S>(p.src, ly, xm, xr, xp, ys[k], ym[k], yp[k], w, out[k]);
+            } else {
+                pull32<S>(p.src, ly, xm, xr, xp, ys[k], ym[k], yp[k], out[k]);
+            }
+        }
+    };
+    // codes of the rows of step 2 (cm), step 1 (c0), the prefetched row (c1)
+    // and the row after it (c2, loaded one iteration ahead)
+    uint32_t cm[V], c0[V], c1[V], c2[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+        cm[k] = 0;
+        c0[k] = load_code(0, k);
+        c1[k] = load_code(1, k);
+    }
+    C A[V][3], B[V][3], Cc[V][3], D[V];  // y-streamed s1: A = row x-2 (1,5,8), B = row x-1 (0,2,4), Cc = row x-1 (1,5,8), D = row x-1 (3)
+    C pf[V][9];
+    pull_row(0, c0, pf);
+    for (uint32_t it = 0; it <= R + 1; it++) {
+        const uint32_t x = row_at(it);
+        C v[V][9];
+#pragma unroll
+        for (int k = 0; k < V; k++)
+#pragma unroll
+            for (int i = 0; i < 9; i++) v[k][i] = pf[k][i];
+        if (it <= R) {
+            pull_row(it + 1, c1, pf);
+#pragma unroll
+            for (int k = 0; k < V; k++) c2[k] = it + 2 <= R + 1 ? load_code(it + 2, k) : 0u;
+        }
+        S *exb = ex + (it & 1) * 6 * (size_t)ly;
+        const S *exp_ = ex + ((it + 1) & 1) * 6 * (size_t)ly;
+        C s013[V][3];
+        // step 1 on row x
+#pragma unroll
+        for (int k = 0; k < V; k++) {
+            if (!live[k]) continue;
+            const uint32_t y = ys[k];
+            uint32_t flags = 0;
+            if (c0[k]) {
+                const ClassEntry &ce = cls[c0[k]];
+                const uint32_t w = ce.src;
+                flags = ce.flags;
+#pragma unroll
+                for (int d = 0; d < 9; d++) {
+                    const uint32_t kind = (w >> (2 * d)) & 3u;
+                    if (kind == kSrcReflect) {
+                        v[k][d] = v[k][d] - (C)ce.k[d];  // v holds the node's own opposite population
+                    } else if (kind == kSrcConst) {
+                        if constexpr (Io<S>::kDev) v[k][d] = (C)(ce.k[d] - kWd[d]);
+                        else v[k][d] = ce.k[d];
+                    } else if (kind == kSrcOutlet) {
+                        v[k][d] = p.side_prev[(size_t)y * 3 + outlet_slot(d)];
+                    }
+                }
+            }
+            NodeOut<S> o;
+            collide_node<S>(v[k], p.omega, o);
+            if (flags & kFlagOutletSide) {
+                side[(size_t)y * 3 + 0] = v[k][3];
+                side[(size_t)y * 3 + 1] = v[k][6];
+                side[(size_t)y * 3 + 2] = v[k][7];
+            }
+            if constexpr (kProbe) {
+                if (p.probe1 && it >= 1 && it <= R && (int64_t)(x * ly + y) == p.probe_cell) {
+                    p.probe1[0] = o.rho;
+                    p.probe1[1] = (double)o.ux;
+                    p.probe1[2] = (double)o.uy;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 6; j++) lds_put<S>(exb + j * ly + y, o.s[kYM[j]]);
+            s013[k][0] = round_to_storage<S>(o.s[0]);
+            s013[k][1] = round_to_storage<S>(o.s[1]);
+            s013[k][2] = round_to_storage<S>(o.s[3]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < V; k++) {
+            if (!live[k]) continue;
+            const uint32_t y = ys[k];
+            C Y[9];
+            Y[0] = s013[k][0];
+            Y[1] = s013[k][1];
+            Y[3] = s013[k][2];
+            Y[2] = lds_get<S>(exb + 0 * ly + ym[k]);
+            Y[5] = lds_get<S>(exb + 1 * ly + ym[k]);
+            Y[6] = lds_get<S>(exb + 2 * ly + ym[k]);
+            Y[4] = lds_get<S>(exb + 3 * ly + yp[k]);
+            Y[7] = lds_get<S>(exb + 4 * ly + yp[k]);
+            Y[8] = lds_get<S>(exb + 5 * ly + yp[k]);
+            if (it >= 2) {
+                // step 2 on row r = x - 1
+                const uint32_t r = row_at(it - 1);
+                C v2[9];
+                v2[0] = B[k][0];
+                v2[2] = B[k][1];
+                v2[4] = B[k][2];
+                v2[1] = A[k][0];
+                v2[5] = A[k][1];
+                v2[8] = A[k][2];
+                v2[3] = Y[3];
+                v2[6] = Y[6];
+                v2[7] = Y[7];
+                uint32_t flags2 = 0;
+                if (cm[k]) {
+                    // reflections read the node's own s1: directions 0, 1, 3 from
+                    // registers, the y-moving ones from the LDS row this thread
+                    // published one iteration earlier
+                    const ClassEntry &ce = cls[cm[k]];
+                    const uint32_t w = ce.src;
+                    flags2 = ce.flags;
+#pragma unroll
+                    for (int d = 0; d < 9; d++) {
+                        const uint32_t kind = (w >> (2 * d)) & 3u;
+                        if (kind == kSrcReflect) {
+                            const int s = kOpp[d];
+                            const C own = s == 0 ? B[k][0] : s == 1 ? Cc[k][0] : s == 3 ? D[k]
+                                                                                   : lds_get<S>(exp_ + kYSlot[s] * ly + y);
+                            v2[d] = own - (C)ce.k[d];
+                        } else if (kind == kSrcConst) {
+                            if constexpr (Io<S>::kDev) v2[d] = (C)(ce.k[d] - kWd[d]);
+                            else v2[d] = ce.k[d];
+                        } else if (kind == kSrcOutlet) {
+                            v2[d] = side[(size_t)y * 3 + outlet_slot(d)];
+                        }
+                    }
+                }
+                NodeOut<S> o2;
+                collide_node<S>(v2, p.omega, o2);
+                // no pressure fix here, so every direction is stored and the
+                // equilibrium is not kept for a fix-up
+#pragma unroll
+                for (int q = 0; q < 9; q++) Io<S>::st(at32(p.dst, (r * 9 + q) * ly + y), o2.s[q], p.nt != 0);
+                if (flags2 & kFlagOutletSide) {
+                    p.side_next[(size_t)y * 3 + 0] = v2[3];
+                    p.side_next[(size_t)y * 3 + 1] = v2[6];
+                    p.side_next[(size_t)y * 3 + 2] = v2[7];
+                }
+                if constexpr (kProbe) {
+                    if (p.probe2 && (int64_t)(r * ly + y) == p.probe_cell) {
+                        p.probe2[0] = o2.rho;
+                        p.probe2[1] = (double)o2.ux;
+                        p.probe2[2] = (double)o2.uy;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 3; j++) A[k][j] = Cc[k][j];
+            B[k][0] = Y[0];
+            B[k][1] = Y[2];
+            B[k][2] = Y[4];
+            Cc[k][0] = Y[1];
+            Cc[k][1] = Y[5];
+            Cc[k][2] = Y[8];
+            D[k] = Y[3];
+            cm[k] = c0[k];
+            c0[k] = c1[k];
+            c1[k] = c2[k];
+        }
+    }
+}
 
 // ---------------------------------------------------------------------------
 // first collision (from the AoS FP64 step input)
@@ -953,6 +1323,124 @@ hipError_t launch_fused(int storage, const Geom &g, const StateView &st, int src
         case LBM_STORE_F64: return fused_typed<double>(g, st, src, bc, omega, pr, slot, partials, cfg, s, nblocks);
         case LBM_STORE_F32: return fused_typed<float>(g, st, src, bc, omega, pr, slot, partials, cfg, s, nblocks);
         case LBM_STORE_F16: return fused_typed<__half>(g, st, src, bc, omega, pr, slot, partials, cfg, s, nblocks);
+    }
+    return hipErrorInvalidValue;
+}
+
+namespace {
+
+constexpr size_t kLdsLimit = 160 * 1024;
+
+size_t sweep_lds_for(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows) {
+    switch (storage) {
+        case LBM_STORE_F64: return sweep_lds_layout<double>(ly, nclasses, rows).total;
+        case LBM_STORE_F32: return sweep_lds_layout<float>(ly, nclasses, rows).total;
+        default: return sweep_lds_layout<__half>(ly, nclasses, rows).total;
+    }
+}
+
+template <class S, int V, bool kProbe>
+hipError_t sweep_go(const SweepParams<S> &p, unsigned grid, unsigned threads, size_t lds, hipStream_t s) {
+    auto fn = k_sweep2<S, V, kProbe>;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(threads), lds, s, p);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView &bc, double omega,
+                       const ProbeView &pr, int64_t slot1, int64_t slot2, const LaunchCfg &cfg, hipStream_t s) {
+    using C = typename Io<S>::C;
+    SweepParams<S> p;
+    p.src = static_cast<const S *>(st.plane[src]);
+    p.dst = static_cast<S *>(st.plane[src ^ 1]);
+    p.side_prev = static_cast<const C *>(st.side[src]);
+    p.side_next = static_cast<C *>(st.side[src ^ 1]);
+    p.g = g;
+    p.bc = bc;
+    p.omega = omega;
+    const bool probing = pr.cell >= 0 && pr.ring && (slot1 >= 0 || slot2 >= 0);
+    p.probe_cell = probing ? pr.cell : -1;
+    p.probe1 = probing && slot1 >= 0 ? pr.ring + 3 * (slot1 % pr.capacity) : nullptr;
+    p.probe2 = probing && slot2 >= 0 ? pr.ring + 3 * (slot2 % pr.capacity) : nullptr;
+    p.nchunks = (uint32_t)(((uint64_t)g.N + 63) / 64);
+    p.nt = cfg.nt_stores;
+    const int V = g.ly <= 1024 ? 1 : 2;
+    const uint32_t per = (g.ly + V - 1) / V;
+    const unsigned threads = (per + 63) / 64 * 64;
+    int slots = 0;
+    if (cfg.sweep_rows <= 0) {
+        // resident blocks per CU (LDS and registers) times the CU count
+        int dev = 0, cus = 0, per_cu = 0;
+        const size_t lds_est = sweep_lds_layout<S>(g.ly, bc.chunk ? bc.nclasses : 0, 64).total;
+        const void *fn = V == 1 ? reinterpret_cast<const void *>(k_sweep2<S, 1, false>)
+                                : reinterpret_cast<const void *>(k_sweep2<S, sizeof(S) == 8 ? 1 : 2, false>);
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_est) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds_est) == hipSuccess)
+            slots = cus * (per_cu > 0 ? per_cu : 1);
+    }
+    p.rows = (uint32_t)sweep_rows_for(g, cfg, slots);
+    const unsigned grid = (g.lx + p.rows - 1) / p.rows;
+    const size_t lds = sweep_lds_layout<S>(g.ly, bc.chunk ? bc.nclasses : 0, p.rows).total;
+#define LBM_SWEEP(VV)                                                                     \
+    return probing ? sweep_go<S, VV, true>(p, grid, threads, lds, s)                      \
+                   : sweep_go<S, VV, false>(p, grid, threads, lds, s)
+    if constexpr (sizeof(S) == 8) {
+        if (V != 1) return hipErrorInvalidValue;
+        LBM_SWEEP(1);
+    } else {
+        if (V == 1) LBM_SWEEP(1);
+        LBM_SWEEP(2);
+    }
+#undef LBM_SWEEP
+}
+
+}  // namespace
+
+bool sweep_supported(int storage, const Geom &g, const BcView &bc) {
+    if (bc.flags & kProgPressure) return false;
+    if (bc.chunk && bc.nclasses > kSweepMaxClasses) return false;
+    // one column per thread for FP64, two for FP32/FP16: the shapes that fit
+    // the 128 registers of a 1024-thread block without spilling
+    const uint32_t max_ly = storage == LBM_STORE_F64 ? 1024 : 2048;
+    if (g.lx < 3 || g.ly < 1 || g.ly > max_ly) return false;
+    // 9 N sizeof(S) bytes addressed with 32-bit offsets
+    if ((uint64_t)g.N * 9 * (storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2)) >= (1ull << 32))
+        return false;
+    return sweep_lds_for(storage, g.ly, bc.chunk ? bc.nclasses : 0, kSweepMaxRows) <= kLdsLimit;
+}
+
+int sweep_rows_for(const Geom &g, const LaunchCfg &cfg, int slots) {
+    int r;
+    if (cfg.sweep_rows > 0) {
+        r = cfg.sweep_rows;
+    } else {
+        // the fewest whole waves of `slots` resident blocks (the fewer the
+        // blocks, the fewer halo rows are read twice), rows spread evenly over
+        // them; at least 8 rows so that the two halo rows on each side stay a
+        // small fraction of the reads
+        const uint32_t s = slots > 0 ? (uint32_t)slots : 256u;
+        const uint32_t waves = (g.lx + s * kSweepMaxRows - 1) / (s * kSweepMaxRows);
+        const uint32_t w = waves > 0 ? waves : 1u;
+        r = (int)((g.lx + w * s - 1) / (w * s));
+        if (r < 8) r = 8;
+    }
+    if (r > (int)kSweepMaxRows) r = (int)kSweepMaxRows;
+    return r < (int)g.lx ? r : (int)g.lx;
+}
+
+hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int src, const BcView &bc,
+                         double omega, const ProbeView &pr, int64_t slot1, int64_t slot2, const LaunchCfg &cfg,
+                         hipStream_t s) {
+    if (!sweep_supported(storage, g, bc)) return hipErrorInvalidValue;
+    switch (storage) {
+        case LBM_STORE_F64: return sweep_typed<double>(g, st, src, bc, omega, pr, slot1, slot2, cfg, s);
+        case LBM_STORE_F32: return sweep_typed<float>(g, st, src, bc, omega, pr, slot1, slot2, cfg, s);
+        case LBM_STORE_F16: return sweep_typed<__half>(g, st, src, bc, omega, pr, slot1, slot2, cfg, s);
     }
     return hipErrorInvalidValue;
 }

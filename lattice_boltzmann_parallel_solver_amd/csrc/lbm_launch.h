@@ -13,6 +13,7 @@ struct BcView {
     const uint16_t *codes = nullptr;       // class per node, N
     const uint8_t *chunk = nullptr;        // nonzero if any node of a 64-node chunk is special
     const ClassEntry *classes = nullptr;
+    uint32_t nclasses = 0;                 // entries of classes (0 with no program)
     const uint32_t *special = nullptr;     // list of special nodes
     int64_t nspecial = 0;
     double rho_in = 0, rho_out = 0;        // pressure fix
@@ -45,6 +46,8 @@ struct LaunchCfg {
     int cells_per_thread = 0;  // 0: 16-byte kernel (needs ly % (16/sizeof) == 0), else 1, 2, 4
     int xcd_remap = 1;
     int nt_stores = 0;
+    int steps_per_launch = 1;  // 2: two-step row-sweep kernel where supported
+    int sweep_rows = 0;        // output rows per block of the sweep kernel (0: automatic)
 };
 
 // One fused step: planes[src] -> planes[dst], side[src] -> side[dst].
@@ -53,6 +56,22 @@ hipError_t launch_fused(int storage, const Geom &g, const StateView &st, int src
                         double omega, const ProbeView &pr, int64_t probe_slot, double *partials,
                         const LaunchCfg &cfg, hipStream_t s, int *nblocks_out);
 int fused_blocks(const Geom &g, const LaunchCfg &cfg);
+
+// Two fused steps in one launch (row sweep with the intermediate state on
+// chip): planes[src] -> planes[src^1], side[src] -> side[src^1].  probe_slot1
+// and probe_slot2 (< 0: none) receive the samples of the two steps.  Returns
+// hipErrorInvalidValue if sweep_supported() is false.
+hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int src, const BcView &bc,
+                         double omega, const ProbeView &pr, int64_t probe_slot1, int64_t probe_slot2,
+                         const LaunchCfg &cfg, hipStream_t s);
+// The sweep kernel needs no pressure fix, at most 256 boundary classes,
+// lx >= 3, ly <= 2048 (FP32, FP16) or 1024 (FP64), 9 N values addressable with
+// 32-bit byte offsets, and its LDS (12 ly storage values, 3 ly compute values,
+// the class table, one byte per row and wave column) within 160 KiB.
+bool sweep_supported(int storage, const Geom &g, const BcView &bc);
+// Output rows per block of the sweep kernel: cfg.sweep_rows if set, else
+// spread over whole waves of `slots` resident blocks (at least 8 rows).
+int sweep_rows_for(const Geom &g, const LaunchCfg &cfg, int slots);
 // Nodes per thread of the 16-byte kernel for a storage type.
 int vec16_nodes(int storage);
 

@@ -155,6 +155,13 @@ class Lattice:
         16-byte groups), XCD-aware block order, non-temporal stores."""
         check(_lib.load().lbm_set_launch(self._h, int(cells_per_thread), int(bool(xcd_remap)), int(bool(nt_stores))))
 
+    def set_temporal(self, steps_per_launch: int = 2, rows_per_block: int = 0) -> None:
+        """Advance `steps_per_launch` (1 or 2) steps per kernel launch; two steps
+        keep the intermediate state on chip (identical results, half the
+        population traffic).  Raises LbmError where the two-step kernel does not
+        apply (pressure fix, ny too large)."""
+        check(_lib.load().lbm_set_temporal(self._h, int(steps_per_launch), int(rows_per_block)))
+
     # -- state ------------------------------------------------------------------
     def _shape_check(self, f=None, rho=None, u=None):
         if f is not None:
