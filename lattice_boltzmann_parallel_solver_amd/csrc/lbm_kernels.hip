@@ -716,16 +716,17 @@ __device__ __forceinline__ typename Io<S>::C lds_get(const S *p) {
     else return *p;
 }
 
-constexpr uint32_t kSweepMaxClasses = 256;
-constexpr uint32_t kSweepMaxRows = 256;
+constexpr uint32_t kSweepMaxClasses = 128;
+constexpr uint32_t kSweepMaxRows = 128;
 
 __host__ __device__ constexpr size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
 
 // Dynamic LDS of the sweep kernel: the y-exchange rows (two buffers of six
-// planes), the step-1 outlet values, the class table and one "special" byte
-// per (row, wave column) of the block's rows.
+// planes), the step-1 outlet values, the incoming outlet values (the input
+// side buffer), the class table and one "special" byte per (row, wave column)
+// of the block's rows.
 struct SweepLds {
-    size_t side, cls, flag, total;
+    size_t side, side_in, cls, flag, total;
 };
 
 template <class S>
@@ -733,7 +734,8 @@ __host__ __device__ inline SweepLds sweep_lds_layout(uint32_t ly, uint32_t nclas
     using C = typename Io<S>::C;
     SweepLds L;
     L.side = align16(12 * (size_t)ly * sizeof(S));
-    L.cls = align16(L.side + 3 * (size_t)ly * sizeof(C));
+    L.side_in = align16(L.side + 3 * (size_t)ly * sizeof(C));
+    L.cls = align16(L.side_in + 3 * (size_t)ly * sizeof(C));
     L.flag = align16(L.cls + (size_t)nclasses * sizeof(ClassEntry));
     L.total = L.flag + (size_t)(rows + 2) * ((ly + 63) / 64);
     return L;
@@ -756,6 +758,11 @@ __device__ __forceinline__ void pull32_bc(const S *__restrict__ src, uint32_t ly
     }
 }
 
+template <bool B>
+struct Flag {
+    static constexpr bool value = B;
+};
+
 template <class S, int V, bool kProbe>
 __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
     using C = typename Io<S>::C;
@@ -770,18 +777,23 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
     const SweepLds L = sweep_lds_layout<S>(ly, has_bc ? p.bc.nclasses : 0, p.rows);
     S *ex = reinterpret_cast<S *>(smem);
     C *side = reinterpret_cast<C *>(smem + L.side);
+    C *side_in = reinterpret_cast<C *>(smem + L.side_in);
     ClassEntry *cls = reinterpret_cast<ClassEntry *>(smem + L.cls);
     uint8_t *wflag = smem + L.flag;
     // row of iteration it (step-1 row xa - 1 + it, periodic)
     auto row_at = [&](uint32_t it) {
-        uint32_t r = xa + it + lx - 1;
-        while (r >= lx) r -= lx;
-        return r;
+        const uint32_t r = xa + it + lx - 1;
+        return r >= lx ? (r - lx >= lx ? r - 2 * lx : r - lx) : r;
     };
     if (has_bc) {
         const uint32_t words = p.bc.nclasses * (uint32_t)(sizeof(ClassEntry) / 4);
         for (uint32_t i = threadIdx.x; i < words; i += T)
             reinterpret_cast<uint32_t *>(cls)[i] = reinterpret_cast<const uint32_t *>(p.bc.classes)[i];
+        // step 1 reads the outlet values of the previous launch from LDS: only
+        // blocks whose step-1 rows include lx-1 (the last block, and the first
+        // through the periodic row xa-1) need them
+        if ((p.bc.flags & kProgOutlet) && (xa == 0 || xa + R + 1 >= lx))
+            for (uint32_t i = threadIdx.x; i < 3 * ly; i += T) side_in[i] = p.side_prev[i];
         for (uint32_t i = threadIdx.x; i < (R + 2) * nw; i += T) {
             const uint32_t rr = i / nw, j = i - rr * nw;
             const uint32_t n0 = row_at(rr) * ly + 64 * j;
@@ -790,33 +802,32 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
         }
         __syncthreads();
     }
+    // Every lane is active: lanes past the last column repeat column ly-1 with
+    // the same inputs, boundary class and results (their stores write the same
+    // values to the same places).  With no lane-dependent branch around memory
+    // operations the compiler can count the stores issued after the prefetch
+    // and does not wait for them at the top of the loop.
     uint32_t ys[V], ym[V], yp[V], jw[V];
-    bool live[V];
 #pragma unroll
     for (int k = 0; k < V; k++) {
-        ys[k] = threadIdx.x + k * T;
-        live[k] = ys[k] < ly;
+        ys[k] = min(threadIdx.x + k * T, ly - 1);
         ym[k] = ys[k] == 0 ? ly - 1 : ys[k] - 1;
         yp[k] = ys[k] + 1 >= ly ? 0 : ys[k] + 1;
-        jw[k] = (threadIdx.x >> 6) + k * (T >> 6);
+        jw[k] = min((threadIdx.x >> 6) + k * (T >> 6), nw - 1);
     }
     // the wave holding column k has boundary nodes in row iteration rr (wave-uniform)
     auto special = [&](uint32_t rr, int k) -> bool {
-        if (!has_bc || jw[k] >= nw) return false;
+        if (!has_bc) return false;
         return __builtin_amdgcn_readfirstlane(wflag[rr * nw + jw[k]]) != 0;
     };
-    // wave-uniform branch only, and an in-bounds address for every lane: a
-    // per-lane branch would make the compiler copy the loaded value at the
-    // join, i.e. wait for it (and for the prefetch issued before it) at once
     auto load_code = [&](uint32_t rr, int k) -> uint32_t {
         if (!special(rr, k)) return 0;
-        return p.bc.codes[row_at(rr) * ly + (live[k] ? ys[k] : ly - 1)];
+        return p.bc.codes[row_at(rr) * ly + ys[k]];
     };
     auto pull_row = [&](uint32_t rr, const uint32_t code[V], C out[V][9]) {
         const uint32_t xr = row_at(rr), xm = xr == 0 ? lx - 1 : xr - 1, xp = xr + 1 == lx ? 0 : xr + 1;
 #pragma unroll
         for (int k = 0; k < V; k++) {
-            if (!live[k]) continue;
             if (special(rr, k)) {
                 const uint32_t w = code[k] ? cls[code[k]].src : 0u;
                 pull32_bc<S>(p.src, ly, xm, xr, xp, ys[k], ym[k], yp[k], w, out[k]);
@@ -833,18 +844,30 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
         cm[k] = 0;
         c0[k] = load_code(0, k);
         c1[k] = load_code(1, k);
+        c2[k] = 0;
     }
     C A[V][3], B[V][3], Cc[V][3], D[V];  // y-streamed s1: A = row x-2 (1,5,8), B = row x-1 (0,2,4), Cc = row x-1 (1,5,8), D = row x-1 (3)
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+        D[k] = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) A[k][j] = B[k][j] = Cc[k][j] = 0;
+    }
     C pf[V][9];
     pull_row(0, c0, pf);
-    for (uint32_t it = 0; it <= R + 1; it++) {
+
+    // One iteration: step 1 on row x = row_at(it) (prefetching the next row
+    // first if kPf), then, if kS2, step 2 on row x - 1.
+    auto body = [&](uint32_t it, auto pf_flag, auto s2_flag) {
+        constexpr bool kPf = decltype(pf_flag)::value;
+        constexpr bool kS2 = decltype(s2_flag)::value;
         const uint32_t x = row_at(it);
         C v[V][9];
 #pragma unroll
         for (int k = 0; k < V; k++)
 #pragma unroll
             for (int i = 0; i < 9; i++) v[k][i] = pf[k][i];
-        if (it <= R) {
+        if constexpr (kPf) {
             pull_row(it + 1, c1, pf);
 #pragma unroll
             for (int k = 0; k < V; k++) c2[k] = it + 2 <= R + 1 ? load_code(it + 2, k) : 0u;
@@ -855,7 +878,6 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
         // step 1 on row x
 #pragma unroll
         for (int k = 0; k < V; k++) {
-            if (!live[k]) continue;
             const uint32_t y = ys[k];
             uint32_t flags = 0;
             if (c0[k]) {
@@ -871,7 +893,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
                         if constexpr (Io<S>::kDev) v[k][d] = (C)(ce.k[d] - kWd[d]);
                         else v[k][d] = ce.k[d];
                     } else if (kind == kSrcOutlet) {
-                        v[k][d] = p.side_prev[(size_t)y * 3 + outlet_slot(d)];
+                        v[k][d] = side_in[(size_t)y * 3 + outlet_slot(d)];
                     }
                 }
             }
@@ -898,7 +920,6 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < V; k++) {
-            if (!live[k]) continue;
             const uint32_t y = ys[k];
             C Y[9];
             Y[0] = s013[k][0];
@@ -910,7 +931,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
             Y[4] = lds_get<S>(exb + 3 * ly + yp[k]);
             Y[7] = lds_get<S>(exb + 4 * ly + yp[k]);
             Y[8] = lds_get<S>(exb + 5 * ly + yp[k]);
-            if (it >= 2) {
+            if constexpr (kS2) {
                 // step 2 on row r = x - 1
                 const uint32_t r = row_at(it - 1);
                 C v2[9];
@@ -979,7 +1000,21 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
             c0[k] = c1[k];
             c1[k] = c2[k];
         }
-    }
+    };
+    // prologue: step-1 rows xa-1 and xa, prefetching rows xa and xa+1
+    // (R >= 1, so both exist)
+    for (uint32_t it = 0; it < 2; it++) body(it, Flag<true>{}, Flag<false>{});
+    // As many stores on this path as on the loop's back edge (they write row
+    // xa of the output, which the first step 2 below overwrites), so that the
+    // wait for the prefetched row at the loop top excludes the latest stores.
+#pragma unroll
+    for (int k = 0; k < V; k++)
+#pragma unroll
+        for (int q = 0; q < 9; q++) Io<S>::st(at32(p.dst, (xa * 9 + q) * ly + ys[k]), (C)0, false);
+    // steady state: prefetch, step 1, step 2
+    for (uint32_t it = 2; it <= R; it++) body(it, Flag<true>{}, Flag<true>{});
+    // epilogue: the last step-1 row (no prefetch) and the last output row
+    body(R + 1, Flag<false>{}, Flag<true>{});
 }
 
 // ---------------------------------------------------------------------------
