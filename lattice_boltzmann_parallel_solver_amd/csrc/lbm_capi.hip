@@ -117,6 +117,7 @@ void release(lbm_ctx *c) {
         (void)hipFree(c->st.plane[k]);
         (void)hipFree(c->st.side[k]);
     }
+    (void)hipFree(c->st.scratch);
     (void)hipFree(c->stage_f);
     (void)hipFree(c->stage_rho);
     (void)hipFree(c->stage_u);
@@ -209,6 +210,7 @@ int lbm_create(lbm_ctx **out, int nx, int ny, int storage, int device) {
         alloc(&c->st.plane[k], 9 * N * plane_elem(storage));
         alloc(&c->st.side[k], 3 * (size_t)ny * side_elem(storage));
     }
+    alloc(&c->st.scratch, kSweepScratchBytes);
     alloc(reinterpret_cast<void **>(&c->stage_f), 9 * N * sizeof(double));
     alloc(reinterpret_cast<void **>(&c->stage_rho), N * sizeof(double));
     alloc(reinterpret_cast<void **>(&c->stage_u), 2 * N * sizeof(double));

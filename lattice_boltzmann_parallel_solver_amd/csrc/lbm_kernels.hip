@@ -696,6 +696,7 @@ struct SweepParams {
     uint32_t rows;   // output rows per block
     uint32_t nchunks;
     int nt;
+    S *scratch;      // 9 V blockDim.x values, written and never read
 };
 
 template <class S>
@@ -1004,13 +1005,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
     // prologue: step-1 rows xa-1 and xa, prefetching rows xa and xa+1
     // (R >= 1, so both exist)
     for (uint32_t it = 0; it < 2; it++) body(it, Flag<true>{}, Flag<false>{});
-    // As many stores on this path as on the loop's back edge (they write row
-    // xa of the output, which the first step 2 below overwrites), so that the
+    // As many stores on this path as on the loop's back edge, so that the
     // wait for the prefetched row at the loop top excludes the latest stores.
+    // They go to a small scratch area shared by all blocks (it stays in L2).
 #pragma unroll
     for (int k = 0; k < V; k++)
 #pragma unroll
-        for (int q = 0; q < 9; q++) Io<S>::st(at32(p.dst, (xa * 9 + q) * ly + ys[k]), (C)0, false);
+        for (int q = 0; q < 9; q++) Io<S>::st(at32(p.scratch, (k * 9 + q) * T + threadIdx.x), (C)0, false);
     // steady state: prefetch, step 1, step 2
     for (uint32_t it = 2; it <= R; it++) body(it, Flag<true>{}, Flag<true>{});
     // epilogue: the last step-1 row (no prefetch) and the last output row
@@ -1402,9 +1403,11 @@ hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView
     p.probe2 = probing && slot2 >= 0 ? pr.ring + 3 * (slot2 % pr.capacity) : nullptr;
     p.nchunks = (uint32_t)(((uint64_t)g.N + 63) / 64);
     p.nt = cfg.nt_stores;
+    p.scratch = static_cast<S *>(st.scratch);
     const int V = g.ly <= 1024 ? 1 : 2;
     const uint32_t per = (g.ly + V - 1) / V;
     const unsigned threads = (per + 63) / 64 * 64;
+    if (!p.scratch || (size_t)9 * V * threads * sizeof(S) > kSweepScratchBytes) return hipErrorInvalidValue;
     int slots = 0;
     if (cfg.sweep_rows <= 0) {
         // resident blocks per CU (LDS and registers) times the CU count

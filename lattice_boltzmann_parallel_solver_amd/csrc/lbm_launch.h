@@ -34,7 +34,12 @@ Geom make_geom(uint32_t lx, uint32_t ly);
 struct StateView {
     void *plane[2] = {nullptr, nullptr};
     void *side[2] = {nullptr, nullptr};
+    void *scratch = nullptr;  // kSweepScratchBytes of write-only scratch for the sweep kernel
 };
+
+// Write-only scratch of a context: the sweep kernel's entry-path stores land
+// here (18 values per thread of a 1024-thread block at most).
+constexpr size_t kSweepScratchBytes = 18 * 1024 * 8;
 
 struct ProbeView {
     int64_t cell = -1;       // node index, -1 = off

@@ -125,7 +125,7 @@ def run_batched(jobs: Sequence[ChannelJob], steps: int, storage: str = "f32", no
 
 
 # ----------------------------------------------------------------------------- autotuning
-_LATTICE_CACHE: Dict[Tuple[str, int], Tuple[int, bool, bool]] = {}
+_LATTICE_CACHE: Dict[Tuple[str, int], Tuple[int, bool, bool, int]] = {}
 _BATCH_CACHE: Dict[Tuple[str, int], int] = {}
 
 
@@ -150,10 +150,13 @@ def _time_run(lat, omega, steps) -> float:
     return time.perf_counter() - t0
 
 
-def autotune_lattice(lat: Lattice, omega: float, steps: int = 30) -> Tuple[int, bool, bool]:
-    """Fastest launch shape for a lattice of this storage and size bucket,
-    timed on a scratch lattice with the same shape and boundary; applied to
-    `lat` and cached."""
+def autotune_lattice(lat: Lattice, omega: float, steps: int = 30) -> Tuple[int, bool, bool, int]:
+    """Fastest launch shape (nodes per thread, XCD order, non-temporal stores,
+    steps per launch) for a lattice of this storage and size bucket, timed on
+    a scratch lattice with the same shape and boundary; applied to `lat` and
+    cached.  Two steps per launch is tried where the lattice supports it."""
+    from ._lib import LbmError
+
     key = (lat.storage, size_bucket(lat.lx * lat.ly))
     best = _LATTICE_CACHE.get(key)
     if best is None:
@@ -167,11 +170,19 @@ def autotune_lattice(lat: Lattice, omega: float, steps: int = 30) -> Tuple[int, 
         for cand in lattice_candidates(lat.ly, lat.storage):
             scratch.set_launch(*cand)
             _time_run(scratch, omega, 3)
-            timings.append((min(_time_run(scratch, omega, steps) for _ in range(2)), cand))
+            timings.append((min(_time_run(scratch, omega, steps) for _ in range(2)), cand + (1,)))
+        try:
+            scratch.set_launch(1, True, False)
+            scratch.set_temporal(2)
+            _time_run(scratch, omega, 4)
+            timings.append((min(_time_run(scratch, omega, steps) for _ in range(2)), (1, True, False, 2)))
+        except LbmError:
+            pass
         scratch.close()
         best = min(timings)[1]
         _LATTICE_CACHE[key] = best
-    lat.set_launch(*best)
+    lat.set_launch(*best[:3])
+    lat.set_temporal(best[3])
     return best
 
 
