@@ -133,6 +133,24 @@ int lbm_read_probe(lbm_ctx *ctx, double *out, int64_t max, int64_t *count);
 /* FP64 sums of rho, rho*ux, rho*uy over the current state (mass and
  * momentum; tests/test_mass_preservation.py:33).  Synchronous. */
 int lbm_moments_sum(lbm_ctx *ctx, double out[3]);
+/* |u| = sqrt(ux*ux + uy*uy) of every node of the current state, nx*ny
+ * doubles in C order (x slowest) at `out` (LBM_HOST or LBM_DEVICE memory).
+ * The field the reference writes every 100 steps with save_mpiio
+ * (experiments.py:638-640, parallelization_utils.py:203-253).  Synchronous. */
+int lbm_speed(lbm_ctx *ctx, double *out, int where);
+/* Checkpoint of the context's state as it is held on the device (running:
+ * populations in the storage type and the outlet buffer; pending: the FP64
+ * step input), with a 64-byte header (magic "LBMSTAT1", version, nx, ny,
+ * storage, phase, step count, payload size).  Resuming from it continues bit
+ * for bit as if the run had not stopped, in every storage.  The boundary
+ * program and probe are not part of it: load into a context created with the
+ * same shape, storage and boundary.  lbm_checkpoint_bytes gives the size
+ * lbm_checkpoint_save needs; both are synchronous, host memory only.  No
+ * counterpart in the reference (its experiments restart from the initial
+ * state). */
+int lbm_checkpoint_bytes(lbm_ctx *ctx, int64_t *bytes);
+int lbm_checkpoint_save(lbm_ctx *ctx, void *host, int64_t capacity);
+int lbm_checkpoint_load(lbm_ctx *ctx, const void *host, int64_t bytes);
 int lbm_synchronize(lbm_ctx *ctx);
 int lbm_steps_done(const lbm_ctx *ctx, int64_t *steps);
 /* Launch tuning of the fused step: nodes per thread (1, 2 or 4; 0 = aligned

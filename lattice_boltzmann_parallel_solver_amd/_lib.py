@@ -88,6 +88,10 @@ _SIGNATURES = {
     "lbm_read_probe": (_I, [_P, _P, _I64, ctypes.POINTER(_I64)]),
     "lbm_moments_sum": (_I, [_P, _PD]),
     "lbm_synchronize": (_I, [_P]),
+    "lbm_speed": (_I, [_P, _P, _I]),
+    "lbm_checkpoint_bytes": (_I, [_P, ctypes.POINTER(_I64)]),
+    "lbm_checkpoint_save": (_I, [_P, _P, _I64]),
+    "lbm_checkpoint_load": (_I, [_P, _P, _I64]),
     "lbm_steps_done": (_I, [_P, ctypes.POINTER(_I64)]),
     "lbm_set_launch": (_I, [_P, _I, _I, _I]),
     "lbm_set_temporal": (_I, [_P, _I, _I]),

@@ -509,6 +509,165 @@ int lbm_moments_sum(lbm_ctx *c, double out[3]) {
     return LBM_OK;
 }
 
+int lbm_speed(lbm_ctx *c, double *out, int where) {
+    int rc = check_ctx(c, "lbm_speed");
+    if (rc) return rc;
+    if (!out) {
+        set_error("lbm_speed: out is NULL");
+        return LBM_ERR_ARG;
+    }
+    if (c->phase == kEmpty) {
+        set_error("lbm_speed: no state loaded");
+        return LBM_ERR_STATE;
+    }
+    const size_t N = c->g.N;
+    if (c->phase == kRunning) {
+        // staging is scratch while running: velocities into stage_u
+        rc = finalize_into(c, nullptr, nullptr, c->stage_u);
+        if (rc) return rc;
+    }
+    double *dst = where == LBM_DEVICE ? out : c->stage_rho;
+    if (where == LBM_HOST && c->phase == kPending) {
+        // keep the pending input's density: compute into a temporary
+        double *tmp = nullptr;
+        LBM_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&tmp), N * sizeof(double)), "lbm_speed");
+        hipError_t e = launch_speed(c->stage_u, tmp, (int64_t)N, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipFree(tmp);
+        LBM_HIP_CHECK(e, "lbm_speed");
+        return LBM_OK;
+    }
+    LBM_HIP_CHECK(launch_speed(c->stage_u, dst, (int64_t)N, c->stream), "speed kernel");
+    if (where == LBM_HOST)
+        LBM_HIP_CHECK(hipMemcpyAsync(out, dst, N * sizeof(double), hipMemcpyDeviceToHost, c->stream), "lbm_speed");
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_speed");
+    return LBM_OK;
+}
+
+// Checkpoint layout: a 64-byte header, then the state as the context holds it
+// (running: populations of the storage type and the outlet buffer; pending:
+// the FP64 step input f, rho, u).
+namespace {
+struct CkptHeader {
+    char magic[8];
+    int32_t version, nx, ny, storage, phase, reserved;
+    int64_t steps, payload;
+    int64_t pad[2];
+};
+static_assert(sizeof(CkptHeader) == 64, "checkpoint header is 64 bytes");
+constexpr char kCkptMagic[8] = {'L', 'B', 'M', 'S', 'T', 'A', 'T', '1'};
+
+int64_t ckpt_payload(const lbm_ctx *c) {
+    const int64_t N = c->g.N;
+    if (c->phase == kRunning) return 9 * N * (int64_t)plane_elem(c->storage) + 3 * (int64_t)c->ny * (int64_t)side_elem(c->storage);
+    return 12 * N * (int64_t)sizeof(double);
+}
+}  // namespace
+
+int lbm_checkpoint_bytes(lbm_ctx *c, int64_t *bytes) {
+    int rc = check_ctx(c, "lbm_checkpoint_bytes");
+    if (rc) return rc;
+    if (!bytes) {
+        set_error("lbm_checkpoint_bytes: bytes is NULL");
+        return LBM_ERR_ARG;
+    }
+    if (c->phase == kEmpty) {
+        set_error("lbm_checkpoint_bytes: no state loaded");
+        return LBM_ERR_STATE;
+    }
+    *bytes = (int64_t)sizeof(CkptHeader) + ckpt_payload(c);
+    return LBM_OK;
+}
+
+int lbm_checkpoint_save(lbm_ctx *c, void *host, int64_t capacity) {
+    int rc = check_ctx(c, "lbm_checkpoint_save");
+    if (rc) return rc;
+    if (c->phase == kEmpty) {
+        set_error("lbm_checkpoint_save: no state loaded");
+        return LBM_ERR_STATE;
+    }
+    const int64_t need = (int64_t)sizeof(CkptHeader) + ckpt_payload(c);
+    if (!host || capacity < need) {
+        set_error("lbm_checkpoint_save: buffer NULL or smaller than lbm_checkpoint_bytes");
+        return LBM_ERR_ARG;
+    }
+    CkptHeader h{};
+    std::memcpy(h.magic, kCkptMagic, 8);
+    h.version = 1;
+    h.nx = c->nx;
+    h.ny = c->ny;
+    h.storage = c->storage;
+    h.phase = c->phase;
+    h.steps = c->steps;
+    h.payload = ckpt_payload(c);
+    std::memcpy(host, &h, sizeof h);
+    char *dst = static_cast<char *>(host) + sizeof h;
+    const size_t N = c->g.N;
+    if (c->phase == kRunning) {
+        const size_t pb = 9 * N * plane_elem(c->storage), sb = 3 * (size_t)c->ny * side_elem(c->storage);
+        LBM_HIP_CHECK(hipMemcpyAsync(dst, c->st.plane[c->cur], pb, hipMemcpyDeviceToHost, c->stream), "lbm_checkpoint_save");
+        LBM_HIP_CHECK(hipMemcpyAsync(dst + pb, c->st.side[c->cur], sb, hipMemcpyDeviceToHost, c->stream), "lbm_checkpoint_save");
+    } else {
+        LBM_HIP_CHECK(hipMemcpyAsync(dst, c->stage_f, 9 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream), "lbm_checkpoint_save");
+        LBM_HIP_CHECK(hipMemcpyAsync(dst + 9 * N * sizeof(double), c->stage_rho, N * sizeof(double), hipMemcpyDeviceToHost, c->stream), "lbm_checkpoint_save");
+        LBM_HIP_CHECK(hipMemcpyAsync(dst + 10 * N * sizeof(double), c->stage_u, 2 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream), "lbm_checkpoint_save");
+    }
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_checkpoint_save");
+    return LBM_OK;
+}
+
+int lbm_checkpoint_load(lbm_ctx *c, const void *host, int64_t bytes) {
+    int rc = check_ctx(c, "lbm_checkpoint_load");
+    if (rc) return rc;
+    if (!host || bytes < (int64_t)sizeof(CkptHeader)) {
+        set_error("lbm_checkpoint_load: buffer NULL or shorter than the header");
+        return LBM_ERR_ARG;
+    }
+    CkptHeader h;
+    std::memcpy(&h, host, sizeof h);
+    if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1) {
+        set_error("lbm_checkpoint_load: not a checkpoint of this library (bad magic or version)");
+        return LBM_ERR_ARG;
+    }
+    if (h.nx != c->nx || h.ny != c->ny || h.storage != c->storage) {
+        set_error("lbm_checkpoint_load: checkpoint is " + std::to_string(h.nx) + "x" + std::to_string(h.ny) +
+                  " storage " + std::to_string(h.storage) + ", context is " + std::to_string(c->nx) + "x" +
+                  std::to_string(c->ny) + " storage " + std::to_string(c->storage));
+        return LBM_ERR_ARG;
+    }
+    if ((h.phase != kRunning && h.phase != kPending) || h.steps < 0) {
+        set_error("lbm_checkpoint_load: corrupt header (phase or step count)");
+        return LBM_ERR_ARG;
+    }
+    const int saved_phase = c->phase;
+    c->phase = h.phase;
+    const int64_t payload = ckpt_payload(c);
+    c->phase = saved_phase;
+    if (h.payload != payload || bytes < (int64_t)sizeof h + payload) {
+        set_error("lbm_checkpoint_load: payload size does not match the lattice");
+        return LBM_ERR_ARG;
+    }
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_checkpoint_load");
+    const char *src = static_cast<const char *>(host) + sizeof h;
+    const size_t N = c->g.N;
+    if (h.phase == kRunning) {
+        const size_t pb = 9 * N * plane_elem(c->storage), sb = 3 * (size_t)c->ny * side_elem(c->storage);
+        LBM_HIP_CHECK(hipMemcpyAsync(c->st.plane[0], src, pb, hipMemcpyHostToDevice, c->stream), "lbm_checkpoint_load");
+        LBM_HIP_CHECK(hipMemcpyAsync(c->st.side[0], src + pb, sb, hipMemcpyHostToDevice, c->stream), "lbm_checkpoint_load");
+        c->cur = 0;
+    } else {
+        LBM_HIP_CHECK(hipMemcpyAsync(c->stage_f, src, 9 * N * sizeof(double), hipMemcpyHostToDevice, c->stream), "lbm_checkpoint_load");
+        LBM_HIP_CHECK(hipMemcpyAsync(c->stage_rho, src + 9 * N * sizeof(double), N * sizeof(double), hipMemcpyHostToDevice, c->stream), "lbm_checkpoint_load");
+        LBM_HIP_CHECK(hipMemcpyAsync(c->stage_u, src + 10 * N * sizeof(double), 2 * N * sizeof(double), hipMemcpyHostToDevice, c->stream), "lbm_checkpoint_load");
+    }
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_checkpoint_load");
+    c->phase = h.phase;
+    c->steps = h.steps;
+    c->probe_base = c->steps;
+    return LBM_OK;
+}
+
 int lbm_synchronize(lbm_ctx *c) {
     int rc = check_ctx(c, "lbm_synchronize");
     if (rc) return rc;

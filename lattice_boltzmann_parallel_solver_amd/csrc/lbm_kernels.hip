@@ -1158,6 +1158,18 @@ __global__ __launch_bounds__(kBlock) void k_moments_sum_aos(const double *__rest
 // ---------------------------------------------------------------------------
 // stateless AoS kernels (drop-ins for the reference's numpy functions)
 // ---------------------------------------------------------------------------
+// |u| per node from AoS FP64 velocities: sqrt(ux*ux + uy*uy) with separate
+// roundings, as numpy's norm over the last axis of length 2
+// (experiments.py:638-640).
+__global__ void k_speed(const double *__restrict__ u, double *__restrict__ out, int64_t n) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const double ux = u[2 * i], uy = u[2 * i + 1];
+        out[i] = __builtin_sqrt(ux * ux + uy * uy);
+    }
+}
+
 template <class T>
 __device__ __forceinline__ void feq_t(T rho, T ux, T uy, T e[9]) {
 #pragma clang fp contract(off)
@@ -1481,6 +1493,12 @@ hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int sr
         case LBM_STORE_F16: return sweep_typed<__half>(g, st, src, bc, omega, pr, slot1, slot2, cfg, s);
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_speed(const double *u, double *out, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_speed, dim3(grid_for(n)), dim3(kBlock), 0, s, u, out, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_collide_first(int storage, const Geom &g, const double *f, const double *rho, const double *u,

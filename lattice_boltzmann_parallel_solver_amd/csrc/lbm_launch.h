@@ -119,6 +119,9 @@ hipError_t launch_finalize(int storage, const Geom &g, const StateView &st, int 
 hipError_t launch_probe_cell(int storage, const Geom &g, const StateView &st, int src, const BcView &bc,
                              const ProbeView &pr, int64_t slot, hipStream_t s);
 
+// |u| = sqrt(ux*ux + uy*uy) of n AoS FP64 velocities.
+hipError_t launch_speed(const double *u, double *out, int64_t n, hipStream_t s);
+
 // Sum nblocks rows of 3 partials into out[3] in a fixed order.
 hipError_t launch_sum_partials(const double *partials, int nblocks, double *out, hipStream_t s);
 // Moment sums of AoS FP64 populations (nblocks_out partial rows).

@@ -215,6 +215,37 @@ class Lattice:
                                         None if u is None else _ptr(u), _lib.HOST))
         return f, rho, u
 
+    def speed(self) -> np.ndarray:
+        """|u| of every node of the current state, (lx, ly) float64: the field
+        the reference saves every 100 steps (experiments.py:638-640)."""
+        out = np.empty((self.lx, self.ly))
+        check(_lib.load().lbm_speed(self._h, _ptr(out), _lib.HOST))
+        return out
+
+    def checkpoint(self) -> bytes:
+        """The device state as bytes (64-byte header + payload); see
+        `restore`.  Resuming continues bit for bit in every storage."""
+        L = _lib.load()
+        n = ctypes.c_int64()
+        check(L.lbm_checkpoint_bytes(self._h, ctypes.byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        check(L.lbm_checkpoint_save(self._h, buf.ctypes.data, n.value))
+        return buf.tobytes()
+
+    def restore(self, blob) -> None:
+        """Load a `checkpoint` taken from a lattice of the same shape and
+        storage (the boundary and probe are this lattice's own)."""
+        buf = np.frombuffer(bytes(blob), dtype=np.uint8)
+        check(_lib.load().lbm_checkpoint_load(self._h, buf.ctypes.data, buf.size))
+
+    def save_checkpoint(self, path: str) -> None:
+        with open(path, "wb") as fh:
+            fh.write(self.checkpoint())
+
+    def load_checkpoint(self, path: str) -> None:
+        with open(path, "rb") as fh:
+            self.restore(fh.read())
+
     def state_into(self, f=None, rho=None, u=None) -> None:
         """Write (f, rho, u) into float64 device tensors (any may be None)."""
         import torch
