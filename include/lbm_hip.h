@@ -183,6 +183,13 @@ int lbm_batch_destroy(lbm_batch *batch);
 int lbm_batch_set_launch(lbm_batch *batch, int nodes_per_thread, int use_graph);
 /* Advance every context of the batch nsteps reference steps (contexts still
  * holding a loaded state start with their first collision).  Synchronous. */
+/* Two steps per launch for the batch (steps_per_launch = 2): every context
+ * the two-step sweep kernel supports (no pressure fix, ny <= 2048 FP32/FP16 or
+ * 1024 FP64) is advanced by it, grouped by block width, `rows_per_block` rows
+ * per block (0 = 8; at least 32 for lattices wide enough for blocks of 512 or
+ * more threads); the others keep the one-step batched kernel.  Results
+ * are identical either way.  Default 1.  No counterpart in the reference. */
+int lbm_batch_set_temporal(lbm_batch *batch, int steps_per_launch, int rows_per_block);
 int lbm_batch_run(lbm_batch *batch, int64_t nsteps);
 
 /* ---- stateless drop-ins on device arrays (AoS, C order) ----------------- */

@@ -98,6 +98,7 @@ _SIGNATURES = {
     "lbm_batch_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double), _I]),
     "lbm_batch_destroy": (_I, [_P]),
     "lbm_batch_set_launch": (_I, [_P, _I, _I]),
+    "lbm_batch_set_temporal": (_I, [_P, _I, _I]),
     "lbm_batch_run": (_I, [_P, _I64]),
     "lbm_stream": (_I, [_P, _P, _I, _I, _I, _P]),
     "lbm_equilibrium": (_I, [_P, _P, _P, _I64, _I, _P]),

@@ -706,6 +706,11 @@ struct lbm_batch {
     hipGraphExec_t exec = nullptr;
     unsigned graph_blocks = 0;
     int graph_v = 0;
+    // two steps per launch for the jobs the sweep kernel supports
+    int temporal = 1;
+    int sweep_rows = 8;
+    void *d_sweep = nullptr;  // sweep job tables and block tables of the last run
+    size_t sweep_cap = 0;
 };
 
 namespace {
@@ -727,7 +732,204 @@ void release_batch(lbm_batch *b) {
     (void)hipFree(b->d_block_job);
     (void)hipFree(b->d_step);
     (void)hipFree(b->d_done);
+    (void)hipFree(b->d_sweep);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+// Steps of a batch with the two-step sweep kernel: jobs it supports are
+// grouped by block width (one launch per group per step pair), the others
+// take two one-step batched launches per pair.  Graph-captured pairs of pairs
+// replay the whole run; an odd last step goes job by job.
+int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
+    const int n = (int)b->ctxs.size();
+    struct Bucket {
+        int v = 1;
+        unsigned threads = 0;
+        size_t lds = 0;
+        std::vector<int> members;
+        std::vector<uint32_t> block_job;  // index into members
+        size_t off_jobs[2] = {0, 0}, off_blocks = 0;
+    };
+    std::vector<Bucket> buckets;
+    std::vector<int> fjobs;  // one-step jobs
+    for (int j = 0; j < n; j++) {
+        lbm_ctx *c = b->ctxs[j];
+        if (!c->st.scratch || !sweep_supported(c->storage, c->g, c->bc.view)) {
+            fjobs.push_back(j);
+            continue;
+        }
+        const int v = c->g.ly <= 1024 ? 1 : 2;
+        const unsigned threads = (unsigned)(((c->g.ly + v - 1) / v + 63) / 64 * 64);
+        Bucket *bk = nullptr;
+        for (Bucket &x : buckets)
+            if (x.v == v && x.threads == threads) bk = &x;
+        if (!bk) {
+            buckets.emplace_back();
+            bk = &buckets.back();
+            bk->v = v;
+            bk->threads = threads;
+        }
+        bk->members.push_back(j);
+    }
+    // host tables: sweep jobs per bucket and pair parity, their block tables
+    std::vector<char> host;
+    auto put = [&](const void *data, size_t bytes) {
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + bytes);
+        std::memcpy(host.data() + off, data, bytes);
+        return off;
+    };
+    for (Bucket &bk : buckets) {
+        std::vector<SweepJob> jobs[2];
+        for (size_t m = 0; m < bk.members.size(); m++) {
+            lbm_ctx *c = b->ctxs[bk.members[m]];
+            // short rows (blocks of a few waves) need many blocks in flight:
+            // few rows per block; wide rows (512+ threads per block) take 32
+            LaunchCfg cfg = c->cfg;
+            cfg.sweep_rows = bk.threads >= 512 ? std::max(b->sweep_rows, 32) : b->sweep_rows;
+            const uint32_t rows = (uint32_t)sweep_rows_for(c->g, cfg, 0);
+            const uint32_t nb = (c->g.lx + rows - 1) / rows;
+            const uint32_t block0 = (uint32_t)bk.block_job.size();
+            bk.block_job.insert(bk.block_job.end(), nb, (uint32_t)m);
+            bk.lds = std::max(bk.lds, sweep_lds_bytes(c->storage, c->g.ly, c->bc.view.chunk ? c->bc.view.nclasses : 0,
+                                                      rows));
+            for (int pp = 0; pp < 2; pp++) {
+                SweepJob d;
+                const int src = c->cur ^ pp;
+                d.src = c->st.plane[src];
+                d.dst = c->st.plane[src ^ 1];
+                d.side_prev = c->st.side[src];
+                d.side_next = c->st.side[src ^ 1];
+                d.scratch = c->st.scratch;
+                d.g = c->g;
+                d.bc = c->bc.view;
+                d.omega = b->omegas[bk.members[m]];
+                d.rows = rows;
+                d.nchunks = (uint32_t)(((uint64_t)c->g.N + 63) / 64);
+                d.block0 = block0;
+                if (c->probe.cell >= 0) {
+                    d.probe_cell = c->probe.cell;
+                    d.probe_ring = c->probe.ring;
+                    d.probe_cap = c->probe.capacity;
+                    d.probe_offset = c->steps - c->probe_base - 1 - b->step_host;
+                }
+                jobs[pp].push_back(d);
+            }
+        }
+        for (int pp = 0; pp < 2; pp++) bk.off_jobs[pp] = put(jobs[pp].data(), jobs[pp].size() * sizeof(SweepJob));
+        bk.off_blocks = put(bk.block_job.data(), bk.block_job.size() * sizeof(uint32_t));
+    }
+    // one-step jobs: tables for the two steps of a pair
+    std::vector<BatchJob> fj[2];
+    std::vector<uint32_t> fblocks;
+    const int per_block = batch_block_nodes(b->v);
+    for (size_t m = 0; m < fjobs.size(); m++) {
+        lbm_ctx *c = b->ctxs[fjobs[m]];
+        const uint32_t nb = (uint32_t)(((int64_t)c->g.N + per_block - 1) / per_block);
+        const uint32_t block0 = (uint32_t)fblocks.size();
+        fblocks.insert(fblocks.end(), nb, (uint32_t)m);
+        for (int p = 0; p < 2; p++) {
+            BatchJob d;
+            const int src = c->cur ^ p;
+            d.src = c->st.plane[src];
+            d.dst = c->st.plane[src ^ 1];
+            d.side_prev = c->st.side[src];
+            d.side_next = c->st.side[src ^ 1];
+            d.g = c->g;
+            d.bc = c->bc.view;
+            d.omega = b->omegas[fjobs[m]];
+            d.block0 = block0;
+            if (c->probe.cell >= 0) {
+                d.probe_cell = c->probe.cell;
+                d.probe_ring = c->probe.ring;
+                d.probe_cap = c->probe.capacity;
+                d.probe_offset = c->steps - c->probe_base - 1 - b->step_host;
+            }
+            fj[p].push_back(d);
+        }
+    }
+    size_t off_f[2] = {0, 0}, off_fb = 0;
+    if (!fjobs.empty()) {
+        for (int p = 0; p < 2; p++) off_f[p] = put(fj[p].data(), fj[p].size() * sizeof(BatchJob));
+        off_fb = put(fblocks.data(), fblocks.size() * sizeof(uint32_t));
+    }
+    for (const Bucket &bk : buckets)
+        LBM_HIP_CHECK(prepare_sweep2_batch(b->storage, bk.v, bk.lds), "lbm_batch_run: sweep kernel LDS");
+    drop_graph(b);  // the one-step graph of an earlier run would hold stale tables
+    if (host.size() > b->sweep_cap) {
+        (void)hipFree(b->d_sweep);
+        b->d_sweep = nullptr;
+        b->sweep_cap = 0;
+        LBM_HIP_CHECK(hipMalloc(&b->d_sweep, host.size()), "lbm_batch_run: sweep tables");
+        b->sweep_cap = host.size();
+    }
+    char *dev = static_cast<char *>(b->d_sweep);
+    if (!host.empty())
+        LBM_HIP_CHECK(hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, b->stream),
+                      "lbm_batch_run: sweep tables");
+    auto pair = [&](int pp) -> hipError_t {
+        for (const Bucket &bk : buckets) {
+            hipError_t e = launch_sweep2_batch(b->storage, bk.v, reinterpret_cast<const SweepJob *>(dev + bk.off_jobs[pp]),
+                                               reinterpret_cast<const uint32_t *>(dev + bk.off_blocks),
+                                               (unsigned)bk.block_job.size(), bk.threads, bk.lds, b->d_step, b->stream);
+            if (e != hipSuccess) return e;
+        }
+        if (!fjobs.empty()) {
+            for (int p = 0; p < 2; p++) {
+                hipError_t e = launch_fused_batch(b->storage, b->v, reinterpret_cast<const BatchJob *>(dev + off_f[p]),
+                                                  reinterpret_cast<const uint32_t *>(dev + off_fb),
+                                                  (unsigned)fblocks.size(), b->d_step, b->d_done, b->stream);
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
+        }
+        return launch_add_step(b->d_step, 2, b->stream);
+    };
+    const int64_t pairs = nsteps / 2;
+    int64_t done_pairs = 0;
+    if (b->use_graph && pairs >= 2) {
+        LBM_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        hipError_t e0 = pair(0);
+        hipError_t e1 = e0 == hipSuccess ? pair(1) : e0;
+        hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
+        if (e0 != hipSuccess || e1 != hipSuccess || ec != hipSuccess) {
+            set_error("lbm_batch_run: graph capture of the sweep pairs failed");
+            drop_graph(b);
+            return LBM_ERR_HIP;
+        }
+        LBM_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+        for (int64_t k = 0; k < pairs / 2; k++) LBM_HIP_CHECK(hipGraphLaunch(b->exec, b->stream), "graph launch");
+        done_pairs = pairs / 2 * 2;
+        drop_graph(b);  // it refers to this run's tables only
+    }
+    for (; done_pairs < pairs; done_pairs++) LBM_HIP_CHECK(pair((int)(done_pairs & 1)), "batched sweep pair");
+    b->step_host += 2 * pairs;
+    for (const Bucket &bk : buckets)
+        for (int j : bk.members) b->ctxs[j]->cur ^= (int)(pairs & 1);
+    for (int j = 0; j < n; j++) b->ctxs[j]->steps += 2 * pairs;
+    if (nsteps & 1) {
+        // the odd step, job by job
+        for (int j = 0; j < n; j++) {
+            lbm_ctx *c = b->ctxs[j];
+            const int64_t slot = c->probe.cell >= 0 ? c->steps - c->probe_base - 1 : -1;
+            LBM_HIP_CHECK(launch_fused(c->storage, c->g, c->st, c->cur, c->bc.view, b->omegas[j], c->probe, slot,
+                                       nullptr, c->cfg, b->stream, nullptr),
+                          "lbm_batch_run: odd step");
+            c->cur ^= 1;
+            c->steps++;
+        }
+        LBM_HIP_CHECK(launch_add_step(b->d_step, 1, b->stream), "lbm_batch_run: step counter");
+        b->step_host += 1;
+    }
+    for (int j = 0; j < n; j++) {
+        lbm_ctx *c = b->ctxs[j];
+        if (c->probe.cell >= 0 && c->steps > c->probe_base)
+            LBM_HIP_CHECK(launch_probe_cell(c->storage, c->g, c->st, c->cur, c->bc.view, c->probe,
+                                            c->steps - c->probe_base - 1, b->stream),
+                          "probe kernel");
+    }
+    LBM_HIP_CHECK(hipStreamSynchronize(b->stream), "lbm_batch_run");
+    return LBM_OK;
 }
 
 }  // namespace
@@ -798,6 +1000,17 @@ int lbm_batch_set_launch(lbm_batch *b, int v, int use_graph) {
     return LBM_OK;
 }
 
+int lbm_batch_set_temporal(lbm_batch *b, int steps_per_launch, int rows_per_block) {
+    if (!b || (steps_per_launch != 1 && steps_per_launch != 2) || rows_per_block < 0) {
+        set_error("lbm_batch_set_temporal: steps_per_launch must be 1 or 2 and rows_per_block >= 0");
+        return LBM_ERR_ARG;
+    }
+    b->temporal = steps_per_launch;
+    b->sweep_rows = rows_per_block > 0 ? rows_per_block : 8;
+    drop_graph(b);
+    return LBM_OK;
+}
+
 int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
     if (!b) {
         set_error("lbm_batch_run: batch is NULL");
@@ -842,6 +1055,7 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
         }
         nsteps--;
     }
+    if (b->temporal == 2) return batch_run_sweep(b, nsteps);
 
     // job tables for the two step parities of this run
     std::vector<BatchJob> jobs[2];

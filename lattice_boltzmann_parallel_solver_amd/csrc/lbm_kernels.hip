@@ -696,6 +696,7 @@ struct SweepParams {
     uint32_t rows;   // output rows per block
     uint32_t nchunks;
     int nt;
+    int xcd;         // XCD-aware block order (neighbouring row ranges share an L2)
     S *scratch;      // 9 V blockDim.x values, written and never read
 };
 
@@ -764,14 +765,14 @@ struct Flag {
     static constexpr bool value = B;
 };
 
+// The rows [xa, xa + p.rows) of one lattice (clipped at lx), two steps.
 template <class S, int V, bool kProbe>
-__global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
+__device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa) {
     using C = typename Io<S>::C;
     extern __shared__ __align__(16) unsigned char smem[];
     const Geom &g = p.g;
     const uint32_t ly = g.ly, lx = g.lx;
     const uint32_t T = blockDim.x;
-    const uint32_t xa = blockIdx.x * p.rows;
     const uint32_t R = min(p.rows, lx - xa);
     const uint32_t nw = (ly + 63) / 64;
     const bool has_bc = p.bc.chunk != nullptr;
@@ -1017,6 +1018,48 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
     // epilogue: the last step-1 row (no prefetch) and the last output row
     body(R + 1, Flag<false>{}, Flag<true>{});
 }
+
+template <class S, int V, bool kProbe>
+__global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
+    const uint32_t blk = p.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    sweep_block<S, V, kProbe>(p, blk * p.rows);
+}
+
+// Two steps of every job of a batch bucket: block b sweeps rows of job
+// block_job[b]; probe samples go to ring slots *step + offset (+1).
+template <class S, int V>
+__global__ __launch_bounds__(kSweepThreads) void k_sweep2_batch(const SweepJob *__restrict__ jobs,
+                                                                const uint32_t *__restrict__ block_job,
+                                                                const int64_t *__restrict__ step) {
+    using C = typename Io<S>::C;
+    // consecutive blocks (neighbouring row ranges of a job) on one XCD, so the
+    // halo rows they both read can hit that XCD's L2
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const SweepJob &j = jobs[block_job[blk]];
+    SweepParams<S> p;
+    p.src = static_cast<const S *>(j.src);
+    p.dst = static_cast<S *>(j.dst);
+    p.side_prev = static_cast<const C *>(j.side_prev);
+    p.side_next = static_cast<C *>(j.side_next);
+    p.g = j.g;
+    p.bc = j.bc;
+    p.omega = j.omega;
+    p.probe_cell = j.probe_cell;
+    p.probe1 = p.probe2 = nullptr;
+    if (j.probe_cell >= 0) {
+        const int64_t slot = *step + j.probe_offset;
+        if (slot >= 0) p.probe1 = j.probe_ring + 3 * (slot % j.probe_cap);
+        if (slot + 1 >= 0) p.probe2 = j.probe_ring + 3 * ((slot + 1) % j.probe_cap);
+    }
+    p.rows = j.rows;
+    p.nchunks = j.nchunks;
+    p.nt = 0;
+    p.xcd = 1;
+    p.scratch = static_cast<S *>(j.scratch);
+    sweep_block<S, V, true>(p, (blk - j.block0) * j.rows);
+}
+
+__global__ void k_add_step(int64_t *step, int64_t k) { *step += k; }
 
 // ---------------------------------------------------------------------------
 // first collision (from the AoS FP64 step input)
@@ -1415,6 +1458,7 @@ hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView
     p.probe2 = probing && slot2 >= 0 ? pr.ring + 3 * (slot2 % pr.capacity) : nullptr;
     p.nchunks = (uint32_t)(((uint64_t)g.N + 63) / 64);
     p.nt = cfg.nt_stores;
+    p.xcd = cfg.xcd_remap;
     p.scratch = static_cast<S *>(st.scratch);
     const int V = g.ly <= 1024 ? 1 : 2;
     const uint32_t per = (g.ly + V - 1) / V;
@@ -1493,6 +1537,59 @@ hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int sr
         case LBM_STORE_F16: return sweep_typed<__half>(g, st, src, bc, omega, pr, slot1, slot2, cfg, s);
     }
     return hipErrorInvalidValue;
+}
+
+namespace {
+template <class S, int V>
+hipError_t sweep_batch_go(const SweepJob *jobs, const uint32_t *block_job, unsigned nblocks, unsigned threads,
+                          size_t lds, const int64_t *step, hipStream_t s) {
+    hipLaunchKernelGGL((k_sweep2_batch<S, V>), dim3(nblocks), dim3(threads), lds, s, jobs, block_job, step);
+    return hipGetLastError();
+}
+
+template <class S, int V>
+hipError_t sweep_batch_attr(size_t lds) {
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(k_sweep2_batch<S, V>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+}  // namespace
+
+hipError_t prepare_sweep2_batch(int storage, int v, size_t lds) {
+    if (lds > kLdsLimit) return hipErrorInvalidValue;
+    switch (storage) {
+        case LBM_STORE_F64: return v == 1 ? sweep_batch_attr<double, 1>(lds) : hipErrorInvalidValue;
+        case LBM_STORE_F32: return v == 1 ? sweep_batch_attr<float, 1>(lds) : sweep_batch_attr<float, 2>(lds);
+        case LBM_STORE_F16: return v == 1 ? sweep_batch_attr<__half, 1>(lds) : sweep_batch_attr<__half, 2>(lds);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_sweep2_batch(int storage, int v, const SweepJob *jobs, const uint32_t *block_job, unsigned nblocks,
+                               unsigned threads, size_t lds, const int64_t *step, hipStream_t s) {
+    if (nblocks == 0) return hipSuccess;
+    if (threads == 0 || threads > (unsigned)kSweepThreads || threads % 64 != 0 || lds > kLdsLimit)
+        return hipErrorInvalidValue;
+    switch (storage) {
+        case LBM_STORE_F64:
+            if (v != 1) return hipErrorInvalidValue;
+            return sweep_batch_go<double, 1>(jobs, block_job, nblocks, threads, lds, step, s);
+        case LBM_STORE_F32:
+            return v == 1 ? sweep_batch_go<float, 1>(jobs, block_job, nblocks, threads, lds, step, s)
+                          : sweep_batch_go<float, 2>(jobs, block_job, nblocks, threads, lds, step, s);
+        case LBM_STORE_F16:
+            return v == 1 ? sweep_batch_go<__half, 1>(jobs, block_job, nblocks, threads, lds, step, s)
+                          : sweep_batch_go<__half, 2>(jobs, block_job, nblocks, threads, lds, step, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_add_step(int64_t *step, int64_t k, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_step, dim3(1), dim3(1), 0, s, step, k);
+    return hipGetLastError();
+}
+
+size_t sweep_lds_bytes(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows) {
+    return sweep_lds_for(storage, ly, nclasses, rows);
 }
 
 hipError_t launch_speed(const double *u, double *out, int64_t n, hipStream_t s) {

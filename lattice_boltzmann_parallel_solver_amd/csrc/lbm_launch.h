@@ -104,6 +104,39 @@ hipError_t launch_fused_batch(int storage, int nodes_per_thread, const BatchJob 
                               unsigned nblocks, int64_t *step, unsigned *done, hipStream_t s);
 int batch_block_nodes(int nodes_per_thread);
 
+// One lattice of a batch advanced by the two-step sweep kernel, buffers
+// resolved for the pair parity of the launch (src -> dst).
+struct SweepJob {
+    const void *src = nullptr;
+    void *dst = nullptr;
+    const void *side_prev = nullptr;
+    void *side_next = nullptr;
+    void *scratch = nullptr;
+    Geom g;
+    BcView bc;
+    double omega = 1.0;
+    int64_t probe_cell = -1;
+    double *probe_ring = nullptr;
+    int64_t probe_cap = 0;
+    int64_t probe_offset = 0;  // step-1 ring slot = batch step counter + probe_offset
+    uint32_t rows = 0;         // output rows per block
+    uint32_t nchunks = 0;
+    uint32_t block0 = 0;       // first block of the job in its bucket's grid
+    uint32_t pad = 0;
+};
+
+// Two steps of every job of a bucket (all jobs with `threads` threads per
+// block and v columns per thread; lds = the largest layout among them).
+hipError_t launch_sweep2_batch(int storage, int v, const SweepJob *jobs, const uint32_t *block_job, unsigned nblocks,
+                               unsigned threads, size_t lds, const int64_t *step, hipStream_t s);
+// Allow `lds` bytes of dynamic LDS for the batched sweep kernel (call before
+// launching or capturing it).
+hipError_t prepare_sweep2_batch(int storage, int v, size_t lds);
+// LDS bytes of the sweep kernel for a lattice of this ly, class count and rows per block.
+size_t sweep_lds_bytes(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows);
+// *step += k (one thread).
+hipError_t launch_add_step(int64_t *step, int64_t k, hipStream_t s);
+
 // First collision from AoS FP64 (f, rho, u) into planes[dst] / side[dst].
 hipError_t launch_collide_first(int storage, const Geom &g, const double *f, const double *rho,
                                 const double *u, const StateView &st, int dst, const BcView &bc,

@@ -154,12 +154,14 @@ class Server:
     """Continuous batching with admission by HBM footprint."""
 
     def __init__(self, hbm_budget: int = 160 << 30, chunk_steps: int = 2000, max_active: int = 256,
-                 device: int = 0, nodes_per_thread: int = 1):
+                 device: int = 0, nodes_per_thread: int = 1, temporal: int = 2, progress=None):
         self.hbm_budget = int(hbm_budget)
         self.chunk_steps = int(chunk_steps)
         self.max_active = int(max_active)
         self.device = device
         self.nodes_per_thread = nodes_per_thread
+        self.temporal = temporal  # steps per launch of the batched step (2: sweep kernel where it applies)
+        self.progress = progress  # called after every chunk with (seconds, active, waiting, finished)
         # bring up the HIP runtime and load the kernels before the first job arrives
         warm = Lattice(8, 8, device=device)
         warm.set_equilibrium(np.ones((8, 8)), np.zeros((8, 8, 2)))
@@ -203,12 +205,15 @@ class Server:
                 if batch is not None:
                     batch.close()
                 batch = Batch([active[i][1] for i in ids], [active[i][0].spec.relaxation for i in ids],
-                              nodes_per_thread=self.nodes_per_thread)
+                              nodes_per_thread=self.nodes_per_thread, temporal=self.temporal)
                 members = ids
             k = min(self.chunk_steps, min(left for _, _, left in active.values()))
             tb = time.perf_counter()
             batch.run(k)
             self.busy += time.perf_counter() - tb
+            if self.progress is not None:
+                self.progress(time.perf_counter() - t0, len(active), len(waiting) + len(incoming),
+                              sum(1 for r in records if r.finished == r.finished))
             finished = []
             for i in ids:
                 rec, lat, left = active[i]
@@ -238,3 +243,88 @@ def summarize(records: Sequence[JobRecord], makespan: float) -> dict:
             "makespan_s": makespan, "aggregate_mlups": work / makespan / 1e6,
             "offered_mlups": work / span / 1e6 if span > 0 else None,
             "unstable": sum(1 for r in records if not r.result.get("finite", False))}
+
+
+def accuracy_vs_fp64(spec: JobSpec, result: dict, device: int = 0) -> dict:
+    """Rerun `spec` alone with FP64 populations (the FP64 path computes the
+    oracle's step bit for bit) and compare: von Karman jobs by the probe trace
+    and the Strouhal numbers, boxes by the u_x profile at x = lx/2."""
+    ref_spec = JobSpec(**{**spec.__dict__, "storage": "f64"})
+    lat = ref_spec.start(device)
+    lat.run(ref_spec.relaxation, ref_spec.steps)
+    ref = ref_spec.result(lat)
+    lat.close()
+    out = {"kind": spec.kind, "storage": spec.storage, "steps": spec.steps}
+    if spec.kind == "von_karman":
+        a, b = result["samples"], ref["samples"]
+        out["probe_max_abs_err_uy"] = float(np.abs(a[:, 2] - b[:, 2]).max())
+        out["probe_max_abs_uy_fp64"] = float(np.abs(b[:, 2]).max())
+        for key in ("strouhal_uy", "strouhal_uy_fft"):
+            if key in result and key in ref:
+                out[key] = result[key]
+                out[key + "_fp64"] = ref[key]
+    else:
+        a, b = result["ux_mid"], ref["ux_mid"]
+        out["profile_max_rel_err"] = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    return out
+
+
+def _main(argv=None):
+    """Serving run: `python -m lattice_boltzmann_parallel_solver_amd.serving --jobs 40`
+    serves a seeded Poisson job stream and prints p50/p99 time-to-result,
+    aggregate MLUPS and, for `--check` jobs of each kind, the error against the
+    same job run alone in FP64."""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--jobs", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--rate", type=float, default=50.0)
+    ap.add_argument("--vk", type=int, nargs=2, default=[8192, 2048])
+    ap.add_argument("--square", type=int, nargs=2, default=[128, 2048])
+    ap.add_argument("--budget", type=int, nargs=2, default=[5000, 100000])
+    ap.add_argument("--storage", default="f32")
+    ap.add_argument("--chunk", type=int, default=2000)
+    ap.add_argument("--check", type=int, default=0, help="jobs of each kind to check against FP64")
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    arrivals = c5_jobs(a.jobs, seed=a.seed, rate=a.rate, vk_size=tuple(a.vk), square=tuple(a.square),
+                       budget=tuple(a.budget), storage=a.storage)
+    work = sum(s.nodes * s.steps for _, s in arrivals)
+    print(json.dumps({"jobs": a.jobs, "node_steps": work, "von_karman": sum(s.kind == "von_karman" for _, s in arrivals)}),
+          flush=True)
+    last = [0.0]
+
+    def progress(t, active, waiting, finished):
+        if t - last[0] >= 10.0:
+            last[0] = t
+            print(json.dumps({"t_s": round(t, 1), "active": active, "waiting": waiting, "finished": finished}),
+                  flush=True)
+
+    server = Server(chunk_steps=a.chunk, progress=progress)
+    recs = server.run(arrivals)
+    summary = summarize(recs, server.makespan)
+    summary["busy_s"] = server.busy
+    summary["config"] = {"jobs": a.jobs, "seed": a.seed, "rate": a.rate, "vk": a.vk, "square": a.square,
+                         "budget": a.budget, "storage": a.storage, "chunk": a.chunk}
+    print(json.dumps(summary), flush=True)
+    jobs = [{"id": r.job_id, "kind": r.spec.kind, "size": [r.spec.lx, r.spec.ly], "steps": r.spec.steps,
+             "arrival_s": r.arrival, "admitted_s": r.admitted, "finished_s": r.finished, "latency_s": r.latency,
+             "strouhal_uy": r.result.get("strouhal_uy")} for r in recs]
+    checks = []
+    if a.check:
+        for kind in ("von_karman", "couette", "poiseuille"):
+            for r in [r for r in recs if r.spec.kind == kind][:a.check]:
+                c = accuracy_vs_fp64(r.spec, r.result)
+                c["id"] = r.job_id
+                checks.append(c)
+                print(json.dumps(c), flush=True)
+    out = {"summary": summary, "jobs": jobs, "fp64_checks": checks}
+    if a.out:
+        with open(a.out, "w") as fh:
+            json.dump(out, fh, indent=1)
+
+
+if __name__ == "__main__":
+    _main()

@@ -305,7 +305,7 @@ class Batch:
     relaxation of lattices[i].  Close the batch before its lattices."""
 
     def __init__(self, lattices: Sequence[Lattice], omegas: Sequence[float], nodes_per_thread: int = 1,
-                 use_graph: bool = True):
+                 use_graph: bool = True, temporal: int = 1, rows_per_block: int = 0):
         if len(lattices) != len(omegas) or not lattices:
             raise ValueError("need one omega per lattice and at least one lattice")
         L = _lib.load()
@@ -316,9 +316,15 @@ class Batch:
         check(L.lbm_batch_create(ctypes.byref(h), handles, oms, len(lattices)))
         self._h = h
         self.set_launch(nodes_per_thread, use_graph)
+        self.set_temporal(temporal, rows_per_block)
 
     def set_launch(self, nodes_per_thread: int = 1, use_graph: bool = True) -> None:
         check(_lib.load().lbm_batch_set_launch(self._h, int(nodes_per_thread), int(bool(use_graph))))
+
+    def set_temporal(self, steps_per_launch: int = 2, rows_per_block: int = 0) -> None:
+        """Two steps per launch for every lattice the sweep kernel supports
+        (identical results); the others keep one step per launch."""
+        check(_lib.load().lbm_batch_set_temporal(self._h, int(steps_per_launch), int(rows_per_block)))
 
     def run(self, steps: int) -> None:
         """Advance every lattice `steps` reference steps (synchronous)."""

@@ -80,14 +80,23 @@ def _sync_all(lattices):
         lat.synchronize()
 
 
-def run_sequential(jobs: Sequence[ChannelJob], steps: int, storage: str = "f32", tune: bool = False):
+def run_sequential(jobs: Sequence[ChannelJob], steps: int, storage: str = "f32", tune: bool = False,
+                   temporal: int = 1):
     """One channel at a time (create, run `steps`, finish), as the reference's
-    sweeps do.  Returns (seconds, aggregate MLUPS)."""
+    sweeps do, with one or two steps per launch (or the autotuned launch shape
+    if `tune`).  Returns (seconds, aggregate MLUPS)."""
+    from ._lib import LbmError
+
     total = 0.0
     for job in jobs:
         lat = build_lattice(job, storage)
         if tune:
             autotune_lattice(lat, job.omega)
+        elif temporal == 2:
+            try:
+                lat.set_temporal(2)
+            except LbmError:
+                pass
         lat.synchronize()
         t0 = time.perf_counter()
         lat.run(job.omega, steps)
@@ -98,14 +107,16 @@ def run_sequential(jobs: Sequence[ChannelJob], steps: int, storage: str = "f32",
 
 
 def run_batched(jobs: Sequence[ChannelJob], steps: int, storage: str = "f32", nodes_per_thread: Optional[int] = None,
-                use_graph: bool = True, chunk: int = 1000, keep: bool = False):
-    """All channels in one Batch.  Returns (seconds, aggregate MLUPS, lattices
-    or None)."""
+                use_graph: bool = True, chunk: int = 1000, keep: bool = False, temporal: int = 2,
+                rows_per_block: int = 0):
+    """All channels in one Batch, two steps per launch by default.  Returns
+    (seconds, aggregate MLUPS, lattices or None)."""
     lats = [build_lattice(j, storage) for j in jobs]
     oms = [j.omega for j in jobs]
     if nodes_per_thread is None:
-        nodes_per_thread = autotune_batch(jobs, storage)
-    b = Batch(lats, oms, nodes_per_thread=nodes_per_thread, use_graph=use_graph)
+        nodes_per_thread = autotune_batch(jobs, storage) if temporal == 1 else 1
+    b = Batch(lats, oms, nodes_per_thread=nodes_per_thread, use_graph=use_graph, temporal=temporal,
+              rows_per_block=rows_per_block)
     b.run(1)  # first collisions, tables, graph capture
     _sync_all(lats)
     t0 = time.perf_counter()
@@ -210,3 +221,44 @@ def autotune_batch(jobs: Sequence[ChannelJob], storage: str = "f32", sample: int
         best = min(timings)[1]
         _BATCH_CACHE[key] = best
     return best
+
+
+def _main(argv=None):
+    """Batched channel workload: `python -m lattice_boltzmann_parallel_solver_amd.workloads
+    --jobs 256 --steps 20000` prints aggregate MLUPS of the batch (two steps
+    per launch, and one), and of the same channels solved one at a time."""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--jobs", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--storage", default="f32")
+    ap.add_argument("--sequential-jobs", type=int, default=0, help="solve only the first N one at a time (0: all)")
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    jobs = channel_jobs(a.jobs, seed=a.seed)
+    res = {"jobs": a.jobs, "steps": a.steps, "seed": a.seed, "storage": a.storage,
+           "nodes": sum(j.nodes for j in jobs), "nx_range": [min(j.lx for j in jobs), max(j.lx for j in jobs)]}
+    print(json.dumps(res), flush=True)
+    sec, ml, _ = run_batched(jobs, a.steps, a.storage, temporal=2)
+    res["batched_two_step"] = {"seconds": sec, "mlups": ml}
+    print(json.dumps({"batched_two_step": res["batched_two_step"]}), flush=True)
+    sec, ml, _ = run_batched(jobs, a.steps, a.storage, temporal=1)
+    res["batched_one_step"] = {"seconds": sec, "mlups": ml}
+    print(json.dumps({"batched_one_step": res["batched_one_step"]}), flush=True)
+    seq = jobs[:a.sequential_jobs] if a.sequential_jobs else jobs
+    sec, ml = run_sequential(seq, a.steps, a.storage)
+    res["sequential"] = {"jobs": len(seq), "seconds": sec, "mlups": ml}
+    res["speedup_two_step_vs_sequential"] = res["batched_two_step"]["mlups"] / ml
+    res["speedup_one_step_vs_sequential"] = res["batched_one_step"]["mlups"] / ml
+    text = json.dumps(res, indent=1)
+    print(text, flush=True)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(text)
+
+
+if __name__ == "__main__":
+    _main()

@@ -1,6 +1,8 @@
-"""Batched step (one launch per step for many lattices) against individual
-lattices run one by one: FP64 bit for bit, including graph replays
-interleaved with eager work, mixed pending/running contexts and probes."""
+"""Batched step (one launch per step for many lattices, or two steps per
+launch with the batched sweep kernel) against individual lattices run one by
+one: bit for bit, including graph replays interleaved with eager work, odd
+step counts, mixed pending/running contexts, probes and jobs the sweep kernel
+does not take (the pressure fix)."""
 
 import numpy as np
 import pytest
@@ -8,7 +10,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def make_jobs(storage="f64", seed=0, n=6):
+def make_jobs(storage="f64", seed=0, n=6, pressure=False, wide=False):
     """Uneven channels plus a Couette and a periodic lattice."""
     from lattice_boltzmann_parallel_solver_amd import Lattice
     from lattice_boltzmann_parallel_solver_amd.boundary_utils import (couette_flow_boundary_conditions,
@@ -21,6 +23,8 @@ def make_jobs(storage="f64", seed=0, n=6):
     for k in range(n):
         lx = int(rng.integers(40, 160))
         ly = int(round(3 * lx / 7))
+        if wide and k == 0:
+            lx, ly = 24, 1300  # two columns per thread in the sweep kernel
         d = max(2, int(round(rng.uniform(0.1, 0.4) * ly)))
         re = rng.uniform(40, 200)
         om = 1.0 / (3 * 0.1 * d / re + 0.5)
@@ -31,6 +35,12 @@ def make_jobs(storage="f64", seed=0, n=6):
     jobs.append((33, 21, couette_flow_boundary_conditions(33, 21, 0.05, 1.0), rho, np.zeros((33, 21, 2)), 1.0))
     rho, u = sinusoidal_velocity_x((17, 64), 0.05)
     jobs.append((17, 64, None, rho, u, 0.7))
+    if pressure:
+        from lattice_boltzmann_parallel_solver_amd.boundary_utils import poiseuille_flow_boundary_conditions
+
+        rho = np.ones((30, 14))
+        jobs.append((30, 14, poiseuille_flow_boundary_conditions(30, 14, 1.0015 / 3, 0.9985 / 3), rho,
+                     np.zeros((30, 14, 2)), 1.4))
     lats = []
     for lx, ly, bc, rho, u, om in jobs:
         lat = Lattice(lx, ly, boundary=bc, storage=storage)
@@ -88,7 +98,7 @@ def test_batch_fp32_matches_individual():
     for lat, r, om in zip(lats, ref, oms):
         r.run(om, 40)
         for a, c in zip(lat.state(), r.state()):
-            np.testing.assert_allclose(a, c, rtol=0, atol=1e-6)
+            assert np.array_equal(a, c)
     b.close()
 
 
@@ -105,4 +115,29 @@ def test_batch_errors():
     b = Batch([a], [1.0])
     with pytest.raises(LbmError, match="no state"):
         b.run(3)
+    b.close()
+
+
+@pytest.mark.parametrize("storage,wide", [("f64", False), ("f32", True), ("f16", True)])
+def test_batched_sweep_equals_individual_single_steps(storage, wide):
+    """Two steps per launch for the batch: channels of several block widths,
+    a Couette box, a periodic lattice and a Poiseuille box (pressure fix, so
+    it stays on the one-step batched kernel) against single-step runs."""
+    from lattice_boltzmann_parallel_solver_amd import Batch
+
+    lats, oms = make_jobs(storage=storage, seed=7, pressure=True, wide=wide and storage != "f64")
+    ref, _ = make_jobs(storage=storage, seed=7, pressure=True, wide=wide and storage != "f64")
+    for lat, r in zip(lats, ref):
+        lat.set_probe(lat.lx // 3, lat.ly // 2, 200)
+        r.set_probe(r.lx // 3, r.ly // 2, 200)
+    b = Batch(lats, oms, temporal=2, rows_per_block=5)
+    for chunk in (1, 9, 14, 3, 8):  # pending start, odd remainders, graph replays of step pairs
+        b.run(chunk)
+        for lat, r, om in zip(lats, ref, oms):
+            r.run(om, chunk)
+    for lat, r in zip(lats, ref):
+        assert lat.steps == r.steps
+        assert np.array_equal(lat.probe(), r.probe())
+        for a, c in zip(lat.state(), r.state()):
+            assert np.array_equal(a, c)
     b.close()
