@@ -262,7 +262,10 @@ extern "C" int lbm_bc_compile(lbm_bc_prog **out, int nx, int ny, const lbm_bc_de
     p->codes.assign(N, 0);
     // one flag per 64 nodes, padded to a multiple of 8 so a wave of the
     // 16-byte kernel (up to 8 chunks) reads its flags with one aligned load
-    p->chunk_flags.assign(((N + 63) / 64 + 7) & ~size_t(7), 0);
+    // one byte per 64 nodes, padded to a multiple of 8 plus 8 spare zero bytes:
+    // a wave whose lanes all lie past the last node may still read the flag
+    // of the chunk after the last one
+    p->chunk_flags.assign((((N + 63) / 64 + 7) & ~size_t(7)) + 8, 0);
     p->classes.push_back(ClassEntry{0, 0, {0}});
     p->rho_in = b.rho_in;
     p->rho_out = b.rho_out;
