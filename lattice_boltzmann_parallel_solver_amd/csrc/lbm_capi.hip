@@ -867,23 +867,25 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     if (!host.empty())
         LBM_HIP_CHECK(hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, b->stream),
                       "lbm_batch_run: sweep tables");
+    // A pair, in order on the batch stream: the groups (they read the pair
+    // counter d_step[1]), then the one-step jobs' two launches (they advance
+    // d_step[0] themselves), then the pair counter.
     auto pair = [&](int pp) -> hipError_t {
+        hipError_t e = hipSuccess;
         for (const Bucket &bk : buckets) {
-            hipError_t e = launch_sweep2_batch(b->storage, bk.v, reinterpret_cast<const SweepJob *>(dev + bk.off_jobs[pp]),
-                                               reinterpret_cast<const uint32_t *>(dev + bk.off_blocks),
-                                               (unsigned)bk.block_job.size(), bk.threads, bk.lds, b->d_step, b->stream);
-            if (e != hipSuccess) return e;
+            if (e != hipSuccess) break;
+            e = launch_sweep2_batch(b->storage, bk.v, reinterpret_cast<const SweepJob *>(dev + bk.off_jobs[pp]),
+                                    reinterpret_cast<const uint32_t *>(dev + bk.off_blocks),
+                                    (unsigned)bk.block_job.size(), bk.threads, bk.lds, b->d_step + 1, b->stream);
         }
-        if (!fjobs.empty()) {
-            for (int p = 0; p < 2; p++) {
-                hipError_t e = launch_fused_batch(b->storage, b->v, reinterpret_cast<const BatchJob *>(dev + off_f[p]),
-                                                  reinterpret_cast<const uint32_t *>(dev + off_fb),
-                                                  (unsigned)fblocks.size(), b->d_step, b->d_done, b->stream);
-                if (e != hipSuccess) return e;
-            }
-            return hipSuccess;
-        }
-        return launch_add_step(b->d_step, 2, b->stream);
+        if (e == hipSuccess && !fjobs.empty())
+            for (int p = 0; p < 2 && e == hipSuccess; p++)
+                e = launch_fused_batch(b->storage, b->v, reinterpret_cast<const BatchJob *>(dev + off_f[p]),
+                                       reinterpret_cast<const uint32_t *>(dev + off_fb), (unsigned)fblocks.size(),
+                                       b->d_step, b->d_done, b->stream);
+        if (e == hipSuccess) e = launch_add_step(b->d_step + 1, 2, b->stream);
+        if (e == hipSuccess && fjobs.empty()) e = launch_add_step(b->d_step, 2, b->stream);
+        return e;
     };
     const int64_t pairs = nsteps / 2;
     int64_t done_pairs = 0;
@@ -898,9 +900,12 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             return LBM_ERR_HIP;
         }
         LBM_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+        // it refers to this run's tables only: no later run may replay it, and
+        // it is destroyed after the final synchronize below, never in flight
+        b->graph_blocks = 0;
+        b->graph_v = -1;
         for (int64_t k = 0; k < pairs / 2; k++) LBM_HIP_CHECK(hipGraphLaunch(b->exec, b->stream), "graph launch");
         done_pairs = pairs / 2 * 2;
-        drop_graph(b);  // it refers to this run's tables only
     }
     for (; done_pairs < pairs; done_pairs++) LBM_HIP_CHECK(pair((int)(done_pairs & 1)), "batched sweep pair");
     b->step_host += 2 * pairs;
@@ -919,6 +924,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             c->steps++;
         }
         LBM_HIP_CHECK(launch_add_step(b->d_step, 1, b->stream), "lbm_batch_run: step counter");
+        LBM_HIP_CHECK(launch_add_step(b->d_step + 1, 1, b->stream), "lbm_batch_run: step counter");
         b->step_host += 1;
     }
     for (int j = 0; j < n; j++) {
@@ -929,6 +935,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
                           "probe kernel");
     }
     LBM_HIP_CHECK(hipStreamSynchronize(b->stream), "lbm_batch_run");
+    drop_graph(b);
     return LBM_OK;
 }
 
@@ -969,9 +976,13 @@ int lbm_batch_create(lbm_batch **out, lbm_ctx *const *ctxs, const double *omegas
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&b->d_jobs[0], sizeof(BatchJob) * n);
     if (e == hipSuccess) e = hipMalloc(&b->d_jobs[1], sizeof(BatchJob) * n);
-    if (e == hipSuccess) e = hipMalloc(&b->d_step, sizeof(int64_t));
+    // d_step[0]: step counter of the one-step batched launches (advanced by
+    // them); d_step[1]: the counter the sweep launches of a pair read, advanced
+    // after the pair
+    if (e == hipSuccess) e = hipMalloc(&b->d_step, 2 * sizeof(int64_t));
     if (e == hipSuccess) e = hipMalloc(&b->d_done, sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemset(b->d_step, 0, sizeof(int64_t));
+    if (e == hipSuccess) e = hipMemset(b->d_step, 0, 2 * sizeof(int64_t));
+
     if (e == hipSuccess) e = hipMemset(b->d_done, 0, sizeof(unsigned));
     if (e != hipSuccess) {
         set_error(std::string("lbm_batch_create: ") + hipGetErrorString(e));
@@ -1055,7 +1066,18 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
         }
         nsteps--;
     }
-    if (b->temporal == 2) return batch_run_sweep(b, nsteps);
+    if (b->temporal == 2) {
+        // Two steps per launch pays when the jobs it cannot take (the pressure
+        // fix) are few: their one-step launches run after the groups and only
+        // fill the GPU with their own blocks.  Above 15 % of the nodes the whole
+        // batch takes the one-step launches, which hold every job at once.
+        int64_t all = 0, other = 0;
+        for (lbm_ctx *c : b->ctxs) {
+            all += c->g.N;
+            if (!c->st.scratch || !sweep_supported(c->storage, c->g, c->bc.view)) other += c->g.N;
+        }
+        if (other * 100 <= all * 15) return batch_run_sweep(b, nsteps);
+    }
 
     // job tables for the two step parities of this run
     std::vector<BatchJob> jobs[2];
@@ -1131,6 +1153,8 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
                                          b->d_done, b->stream),
                       "batched step kernel");
     }
+    // keep the counter the two-step launches read in step with this one
+    LBM_HIP_CHECK(launch_add_step(b->d_step + 1, nsteps, b->stream), "lbm_batch_run: step counter");
     b->step_host += nsteps;
     for (int j = 0; j < n; j++) {
         lbm_ctx *c = b->ctxs[j];

@@ -141,3 +141,26 @@ def test_batched_sweep_equals_individual_single_steps(storage, wide):
         for a, c in zip(lat.state(), r.state()):
             assert np.array_equal(a, c)
     b.close()
+
+
+def test_batch_switching_between_one_and_two_steps_per_launch_keeps_probes():
+    """Runs alternating between the one-step and the two-step batched paths
+    keep every job's probe ring slots and state equal to single runs."""
+    from lattice_boltzmann_parallel_solver_amd import Batch
+
+    lats, oms = make_jobs(storage="f32", seed=11, n=4, pressure=True)
+    ref, _ = make_jobs(storage="f32", seed=11, n=4, pressure=True)
+    for lat, r in zip(lats, ref):
+        lat.set_probe(lat.lx // 2, lat.ly // 3, 300)
+        r.set_probe(r.lx // 2, r.ly // 3, 300)
+    b = Batch(lats, oms)
+    for temporal, chunk in ((1, 5), (2, 12), (1, 7), (2, 9), (2, 4)):
+        b.set_temporal(temporal)
+        b.run(chunk)
+        for r, om in zip(ref, oms):
+            r.run(om, chunk)
+    for lat, r in zip(lats, ref):
+        assert np.array_equal(lat.probe(), r.probe())
+        for a, c in zip(lat.state(), r.state()):
+            assert np.array_equal(a, c)
+    b.close()
