@@ -115,16 +115,20 @@ def serving_leg(n=24, seed=0):
     return s
 
 
-def batching_leg(n=64, steps=1000, seed=0):
-    """Seeded uneven channels (nx U[200,1200], ny = 3nx/7, blockage U[0.1,0.4],
-    Re U[40,200], FP32): one launch per step for all versus one at a time."""
+def batching_leg(n=256, steps=1000, seed=0):
+    """The 256 seeded uneven channels of the batching workload (nx U[200,1200],
+    ny = 3nx/7, blockage U[0.1,0.4], Re U[40,200], FP32), 1000 steps: one
+    batch (two steps per launch) versus the same channels solved one at a
+    time (one step per launch, their fastest form)."""
     from lattice_boltzmann_parallel_solver_amd import workloads as W
 
     jobs = W.channel_jobs(n, seed=seed)
-    _, batched, _ = W.run_batched(jobs, steps, "f32")
+    _, batched, _ = W.run_batched(jobs, steps, "f32", temporal=2)
     _, sequential = W.run_sequential(jobs, steps, "f32")
     return {"jobs": n, "steps": steps, "nodes": sum(j.nodes for j in jobs), "batched_mlups": batched,
-            "sequential_mlups": sequential, "speedup": batched / sequential}
+            "sequential_mlups": sequential, "speedup": batched / sequential,
+            "workload": f"{n} channels, seed {seed}, {steps} steps, f32; full-length run (20k steps) in "
+                        "profiles/batching_c4_256_channels.json"}
 
 
 def cpu_baseline(steps=2, mp_steps=6, c_steps=10):
