@@ -10,7 +10,8 @@ single-lattice path, for comparison.
 ``autotune_lattice`` / ``autotune_batch`` pick the launch shape of the fused
 step per size bucket by timing the candidates on scratch lattices of the same
 shape and boundary (the real state is never advanced by tuning); results are
-cached per (storage, size bucket).
+cached per (storage, size bucket), and for single lattices also per whether ny
+allows 16-byte groups.
 """
 
 from __future__ import annotations
@@ -136,7 +137,7 @@ def run_batched(jobs: Sequence[ChannelJob], steps: int, storage: str = "f32", no
 
 
 # ----------------------------------------------------------------------------- autotuning
-_LATTICE_CACHE: Dict[Tuple[str, int], Tuple[int, bool, bool, int]] = {}
+_LATTICE_CACHE: Dict[Tuple[str, int, bool], Tuple[int, bool, bool, int, int]] = {}
 _BATCH_CACHE: Dict[Tuple[str, int], int] = {}
 
 
@@ -161,14 +162,19 @@ def _time_run(lat, omega, steps) -> float:
     return time.perf_counter() - t0
 
 
-def autotune_lattice(lat: Lattice, omega: float, steps: int = 30) -> Tuple[int, bool, bool, int]:
+def autotune_lattice(lat: Lattice, omega: float, steps: int = 30) -> Tuple[int, bool, bool, int, int]:
     """Fastest launch shape (nodes per thread, XCD order, non-temporal stores,
-    steps per launch) for a lattice of this storage and size bucket, timed on
-    a scratch lattice with the same shape and boundary; applied to `lat` and
-    cached.  Two steps per launch is tried where the lattice supports it."""
+    steps per launch, rows per block of the two-step sweep) for a lattice of
+    this storage and size bucket, timed on a scratch lattice with the same
+    shape and boundary; applied to `lat` and cached.  Two steps per launch is
+    tried where the lattice supports it, with the occupancy-derived row count
+    (0) and with 1, 2 and 4 rows per block: lattices of a few hundred rows
+    fill the chip only with very short blocks."""
     from ._lib import LbmError
 
-    key = (lat.storage, size_bucket(lat.lx * lat.ly))
+    # 16-byte groups exist only where ny is a multiple of their node count, so
+    # lattices with and without that property are tuned apart
+    key = (lat.storage, size_bucket(lat.lx * lat.ly), lat.ly % {"f64": 2, "f32": 4, "f16": 8}[lat.storage] == 0)
     best = _LATTICE_CACHE.get(key)
     if best is None:
         scratch = Lattice(lat.lx, lat.ly, boundary=lat._program, storage=lat.storage, device=lat.device)
@@ -181,19 +187,25 @@ def autotune_lattice(lat: Lattice, omega: float, steps: int = 30) -> Tuple[int, 
         for cand in lattice_candidates(lat.ly, lat.storage):
             scratch.set_launch(*cand)
             _time_run(scratch, omega, 3)
-            timings.append((min(_time_run(scratch, omega, steps) for _ in range(2)), cand + (1,)))
-        try:
-            scratch.set_launch(1, True, False)
-            scratch.set_temporal(2)
+            timings.append((min(_time_run(scratch, omega, steps) for _ in range(2)), cand + (1, 0)))
+        scratch.set_launch(1, True, False)
+        for rows in (0, 1, 2, 4):
+            try:
+                scratch.set_temporal(2, rows)
+            except LbmError:
+                break
             _time_run(scratch, omega, 4)
-            timings.append((min(_time_run(scratch, omega, steps) for _ in range(2)), (1, True, False, 2)))
-        except LbmError:
-            pass
+            timings.append((min(_time_run(scratch, omega, steps) for _ in range(2)), (1, True, False, 2, rows)))
         scratch.close()
         best = min(timings)[1]
         _LATTICE_CACHE[key] = best
     lat.set_launch(*best[:3])
-    lat.set_temporal(best[3])
+    try:
+        lat.set_temporal(best[3], best[4])
+    except LbmError:
+        # a lattice of the same size bucket that the two-step kernel does not take
+        best = best[:3] + (1, 0)
+        lat.set_temporal(1)
     return best
 
 
@@ -226,7 +238,8 @@ def autotune_batch(jobs: Sequence[ChannelJob], storage: str = "f32", sample: int
 def _main(argv=None):
     """Batched channel workload: `python -m lattice_boltzmann_parallel_solver_amd.workloads
     --jobs 256 --steps 20000` prints aggregate MLUPS of the batch (two steps
-    per launch, and one), and of the same channels solved one at a time."""
+    per launch, and one), and of the same channels solved one at a time, each
+    with the autotuned launch shape of its size bucket."""
     import argparse
     import json
 
@@ -249,8 +262,11 @@ def _main(argv=None):
     res["batched_one_step"] = {"seconds": sec, "mlups": ml}
     print(json.dumps({"batched_one_step": res["batched_one_step"]}), flush=True)
     seq = jobs[:a.sequential_jobs] if a.sequential_jobs else jobs
-    sec, ml = run_sequential(seq, a.steps, a.storage)
-    res["sequential"] = {"jobs": len(seq), "seconds": sec, "mlups": ml}
+    sec, ml = run_sequential(seq, a.steps, a.storage, tune=True)
+    res["sequential"] = {"jobs": len(seq), "seconds": sec, "mlups": ml,
+                         "launch": "autotuned per size bucket (one step per launch, or two with 0/1/2/4 rows per block)",
+                         "shapes_chosen": {f"{k[0]}/bucket{k[1]}/{'v16' if k[2] else 'ragged'}": list(v)
+                                           for k, v in sorted(_LATTICE_CACHE.items())}}
     res["speedup_two_step_vs_sequential"] = res["batched_two_step"]["mlups"] / ml
     res["speedup_one_step_vs_sequential"] = res["batched_one_step"]["mlups"] / ml
     text = json.dumps(res, indent=1)

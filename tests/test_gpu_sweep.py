@@ -134,13 +134,14 @@ def test_two_steps_rejected_where_unsupported():
 
 def test_autotuned_launch_shape_gives_the_same_run():
     """Every launch shape the autotuner can pick (nodes per thread, 16-byte
-    groups, XCD order, one or two steps per launch) computes the same step."""
+    groups, XCD order, one or two steps per launch, rows per sweep block)
+    computes the same step."""
     from lattice_boltzmann_parallel_solver_amd import workloads as W
 
     job = W.ChannelJob(96, 64, 16, 100.0)
     a = W.build_lattice(job, "f32")
     best = W.autotune_lattice(a, job.omega, steps=6)
-    assert len(best) == 4 and best[3] in (1, 2)
+    assert len(best) == 5 and best[3] in (1, 2) and best[4] in (0, 1, 2, 4)
     b = W.build_lattice(job, "f32")
     a.run(job.omega, 25)
     b.run(job.omega, 25)
