@@ -140,7 +140,7 @@ int lbm_moments_sum(lbm_ctx *ctx, double out[3]);
 int lbm_speed(lbm_ctx *ctx, double *out, int where);
 /* Checkpoint of the context's state as it is held on the device (running:
  * populations in the storage type and the outlet buffer; pending: the FP64
- * step input), with a 64-byte header (magic "LBMSTAT1", version, nx, ny,
+ * step input), with a 64-byte header (magic "LBMSTAT1", version 2, nx, ny,
  * storage, phase, step count, payload size).  Resuming from it continues bit
  * for bit as if the run had not stopped, in every storage.  The boundary
  * program and probe are not part of it: load into a context created with the

@@ -266,6 +266,10 @@ extern "C" int lbm_bc_compile(lbm_bc_prog **out, int nx, int ny, const lbm_bc_de
     // a wave whose lanes all lie past the last node may still read the flag
     // of the chunk after the last one
     p->chunk_flags.assign((((N + 63) / 64 + 7) & ~size_t(7)) + 8, 0);
+    // one byte per row and 64-column wave (the unit of the row-aligned
+    // kernels), plus 8 spare zero bytes
+    const size_t wpr = ((size_t)ny + 63) / 64;
+    p->wave_flags.assign((size_t)nx * wpr + 8, 0);
     p->classes.push_back(ClassEntry{0, 0, {0}});
     p->rho_in = b.rho_in;
     p->rho_out = b.rho_out;
@@ -308,6 +312,7 @@ extern "C" int lbm_bc_compile(lbm_bc_prog **out, int nx, int ny, const lbm_bc_de
         }
         p->codes[node] = code;
         p->chunk_flags[node / 64] = 1;
+        p->wave_flags[(size_t)(node / (uint32_t)ny) * wpr + (node % (uint32_t)ny) / 64] = 1;
         p->special.push_back(node);
     }
     *out = p;

@@ -56,7 +56,8 @@ hipError_t upload_bc(const lbm_bc_prog &p, BcUpload &up) {
     const size_t N = p.codes.size();
     const size_t o_codes = 0;
     const size_t o_chunk = (o_codes + N * sizeof(uint16_t) + 255) & ~size_t(255);
-    const size_t o_cls = (o_chunk + p.chunk_flags.size() + 255) & ~size_t(255);
+    const size_t o_wave = (o_chunk + p.chunk_flags.size() + 255) & ~size_t(255);
+    const size_t o_cls = (o_wave + p.wave_flags.size() + 255) & ~size_t(255);
     const size_t o_spec = (o_cls + p.classes.size() * sizeof(ClassEntry) + 255) & ~size_t(255);
     const size_t total = o_spec + p.special.size() * sizeof(uint32_t) + 256;
     char *mem = nullptr;
@@ -65,6 +66,8 @@ hipError_t upload_bc(const lbm_bc_prog &p, BcUpload &up) {
     e = hipMemcpy(mem + o_codes, p.codes.data(), N * sizeof(uint16_t), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(mem + o_chunk, p.chunk_flags.data(), p.chunk_flags.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(mem + o_wave, p.wave_flags.data(), p.wave_flags.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(mem + o_cls, p.classes.data(), p.classes.size() * sizeof(ClassEntry), hipMemcpyHostToDevice);
     if (e == hipSuccess && !p.special.empty())
@@ -76,6 +79,7 @@ hipError_t upload_bc(const lbm_bc_prog &p, BcUpload &up) {
     up.mem = mem;
     up.view.codes = reinterpret_cast<const uint16_t *>(mem + o_codes);
     up.view.chunk = reinterpret_cast<const uint8_t *>(mem + o_chunk);
+    up.view.wave = reinterpret_cast<const uint8_t *>(mem + o_wave);
     up.view.classes = reinterpret_cast<const ClassEntry *>(mem + o_cls);
     up.view.nclasses = (uint32_t)p.classes.size();
     up.view.special = reinterpret_cast<const uint32_t *>(mem + o_spec);
@@ -109,6 +113,9 @@ struct lbm_ctx {
 };
 
 namespace {
+
+// Bytes of one state buffer: nine planes of lx rows, `pitch` elements a row.
+size_t plane_bytes(const lbm_ctx *c) { return 9 * (size_t)c->g.lx * c->g.pitch * plane_elem(c->storage); }
 
 void release(lbm_ctx *c) {
     (void)hipSetDevice(c->device);
@@ -197,7 +204,7 @@ int lbm_create(lbm_ctx **out, int nx, int ny, int storage, int device) {
     c->ny = ny;
     c->storage = storage;
     c->device = device;
-    c->g = make_geom((uint32_t)nx, (uint32_t)ny);
+    c->g = make_geom((uint32_t)nx, (uint32_t)ny, (uint32_t)(128 / plane_elem(storage)));
     // one node per thread with XCD-aware block order: the fastest shape
     // measured on MI355X for all three storages (8192x2048 channel)
     c->cfg = LaunchCfg{1, 1, 0};
@@ -207,7 +214,7 @@ int lbm_create(lbm_ctx **out, int nx, int ny, int storage, int device) {
         if (e == hipSuccess) e = hipMalloc(p, bytes);
     };
     for (int k = 0; k < 2; k++) {
-        alloc(&c->st.plane[k], 9 * N * plane_elem(storage));
+        alloc(&c->st.plane[k], plane_bytes(c));
         alloc(&c->st.side[k], 3 * (size_t)ny * side_elem(storage));
     }
     alloc(&c->st.scratch, kSweepScratchBytes);
@@ -557,10 +564,12 @@ struct CkptHeader {
 };
 static_assert(sizeof(CkptHeader) == 64, "checkpoint header is 64 bytes");
 constexpr char kCkptMagic[8] = {'L', 'B', 'M', 'S', 'T', 'A', 'T', '1'};
+// 2: plane rows padded to whole 128-byte lines
+constexpr int32_t kCkptVersion = 2;
 
 int64_t ckpt_payload(const lbm_ctx *c) {
     const int64_t N = c->g.N;
-    if (c->phase == kRunning) return 9 * N * (int64_t)plane_elem(c->storage) + 3 * (int64_t)c->ny * (int64_t)side_elem(c->storage);
+    if (c->phase == kRunning) return (int64_t)plane_bytes(c) + 3 * (int64_t)c->ny * (int64_t)side_elem(c->storage);
     return 12 * N * (int64_t)sizeof(double);
 }
 }  // namespace
@@ -594,7 +603,7 @@ int lbm_checkpoint_save(lbm_ctx *c, void *host, int64_t capacity) {
     }
     CkptHeader h{};
     std::memcpy(h.magic, kCkptMagic, 8);
-    h.version = 1;
+    h.version = kCkptVersion;
     h.nx = c->nx;
     h.ny = c->ny;
     h.storage = c->storage;
@@ -605,7 +614,7 @@ int lbm_checkpoint_save(lbm_ctx *c, void *host, int64_t capacity) {
     char *dst = static_cast<char *>(host) + sizeof h;
     const size_t N = c->g.N;
     if (c->phase == kRunning) {
-        const size_t pb = 9 * N * plane_elem(c->storage), sb = 3 * (size_t)c->ny * side_elem(c->storage);
+        const size_t pb = plane_bytes(c), sb = 3 * (size_t)c->ny * side_elem(c->storage);
         LBM_HIP_CHECK(hipMemcpyAsync(dst, c->st.plane[c->cur], pb, hipMemcpyDeviceToHost, c->stream), "lbm_checkpoint_save");
         LBM_HIP_CHECK(hipMemcpyAsync(dst + pb, c->st.side[c->cur], sb, hipMemcpyDeviceToHost, c->stream), "lbm_checkpoint_save");
     } else {
@@ -626,7 +635,7 @@ int lbm_checkpoint_load(lbm_ctx *c, const void *host, int64_t bytes) {
     }
     CkptHeader h;
     std::memcpy(&h, host, sizeof h);
-    if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1) {
+    if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != kCkptVersion) {
         set_error("lbm_checkpoint_load: not a checkpoint of this library (bad magic or version)");
         return LBM_ERR_ARG;
     }
@@ -652,7 +661,7 @@ int lbm_checkpoint_load(lbm_ctx *c, const void *host, int64_t bytes) {
     const char *src = static_cast<const char *>(host) + sizeof h;
     const size_t N = c->g.N;
     if (h.phase == kRunning) {
-        const size_t pb = 9 * N * plane_elem(c->storage), sb = 3 * (size_t)c->ny * side_elem(c->storage);
+        const size_t pb = plane_bytes(c), sb = 3 * (size_t)c->ny * side_elem(c->storage);
         LBM_HIP_CHECK(hipMemcpyAsync(c->st.plane[0], src, pb, hipMemcpyHostToDevice, c->stream), "lbm_checkpoint_load");
         LBM_HIP_CHECK(hipMemcpyAsync(c->st.side[0], src + pb, sb, hipMemcpyHostToDevice, c->stream), "lbm_checkpoint_load");
         c->cur = 0;
@@ -1256,11 +1265,13 @@ static int prog_on_device(const lbm_bc_prog *prog, BcView &view, const char *fn)
         p->dev_id = dev;
         p->dev_codes = up.view.codes;
         p->dev_chunk = up.view.chunk;
+        p->dev_wave = up.view.wave;
         p->dev_classes = up.view.classes;
         p->dev_special = up.view.special;
     }
     view.codes = p->dev_codes;
     view.chunk = p->dev_chunk;
+    view.wave = p->dev_wave;
     view.classes = p->dev_classes;
     view.nclasses = (uint32_t)p->classes.size();
     view.special = p->dev_special;

@@ -55,6 +55,7 @@ struct lbm_bc_prog {
     std::vector<uint16_t> codes;            // nx*ny
     std::vector<lbm::ClassEntry> classes;   // classes[0] = plain fluid
     std::vector<uint8_t> chunk_flags;       // one per 64 linear nodes
+    std::vector<uint8_t> wave_flags;        // one per (row, 64-column wave), x * ceil(ny/64) + y/64
     std::vector<uint32_t> special;          // nodes with a nonzero class
     double rho_in = 0, rho_out = 0;         // pressure fix densities
     uint32_t d1 = 0, d2 = 0;                // pressure fix direction sets
@@ -64,6 +65,7 @@ struct lbm_bc_prog {
     void *dev_mem = nullptr;
     const uint16_t *dev_codes = nullptr;
     const uint8_t *dev_chunk = nullptr;
+    const uint8_t *dev_wave = nullptr;
     const lbm::ClassEntry *dev_classes = nullptr;
     const uint32_t *dev_special = nullptr;
 };

@@ -1,11 +1,14 @@
 // D2Q9 lattice-Boltzmann kernels for gfx950 (MI355X).
 //
 // State layout in HBM: row-interleaved planes, population i of node (x, y) at
-// ((x*9 + i)*ly + y): every row stores its nine y-contiguous planes back to
+// ((x*9 + i)*pitch + y): every row stores its nine y-contiguous planes back to
 // back, so the three source rows of a wavefront are three contiguous blocks
 // (measured faster than nine whole-lattice planes, which stream from nine
-// regions at once).  Node n = x*ly + y (x slowest, as the reference's
-// f[x, y, i]); two buffers (A/B) alternate every step.  The hot kernel is a
+// regions at once).  pitch is ly rounded up to whole 128-byte lines, so every
+// plane row starts on a line (a row length off the line costs the one-step
+// kernel about 5 %: each wave then touches one more line per plane); the pad
+// elements are never read or written.  Node n = x*ly + y (x slowest, as the
+// reference's f[x, y, i]); two buffers (A/B) alternate every step.  The hot kernel is a
 // pull-scheme fused step: each thread gathers the nine populations streaming
 // into its node, applies the node's boundary class, computes the moments,
 // the equilibrium and the BGK relaxation in registers, and writes the nine
@@ -25,12 +28,15 @@
 
 namespace lbm {
 
-Geom make_geom(uint32_t lx, uint32_t ly) {
+Geom make_geom(uint32_t lx, uint32_t ly, uint32_t align) {
     Geom g;
     g.lx = lx;
     g.ly = ly;
     g.N = lx * ly;
+    g.pitch = (ly + align - 1) / align * align;
+    g.wpr = (ly + 63) / 64;
     g.ly_magic = ly == 1 ? 0 : (~0ull) / ly + 1;
+    g.wpr_magic = g.wpr == 1 ? 0 : (~0ull) / g.wpr + 1;
     return g;
 }
 
@@ -114,7 +120,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
 
 // Offset of population i of node (x, y) in a state buffer.
 __device__ __forceinline__ size_t pop(const Geom &g, uint32_t x, int i, uint32_t y) {
-    return ((size_t)x * 9 + i) * g.ly + y;
+    return ((size_t)x * 9 + i) * g.pitch + y;
 }
 
 // Source row of direction i for destination row x (periodic in x).
@@ -141,7 +147,7 @@ __device__ __forceinline__ void pull(const S *__restrict__ src, const Geom &g, u
 
 // Element `off` of a state buffer through a 32-bit byte offset (lets the
 // compiler use the uniform-base + 32-bit-offset addressing form).  Valid while
-// 9 * N * sizeof(S) < 2^32.
+// 9 * lx * pitch * sizeof(S) < 2^32.
 template <class S>
 __device__ __forceinline__ const S *at32(const S *base, uint32_t off) {
     return reinterpret_cast<const S *>(reinterpret_cast<const char *>(base) + (uint32_t)(off * (uint32_t)sizeof(S)));
@@ -153,14 +159,14 @@ __device__ __forceinline__ S *at32(S *base, uint32_t off) {
 
 // pull() with 32-bit offsets: the three source rows are given.
 template <class S>
-__device__ __forceinline__ void pull32(const S *__restrict__ src, uint32_t ly, uint32_t xm, uint32_t x,
+__device__ __forceinline__ void pull32(const S *__restrict__ src, uint32_t pitch, uint32_t xm, uint32_t x,
                                        uint32_t xp, uint32_t y, uint32_t ym, uint32_t yp,
                                        typename Io<S>::C out[9]) {
 #pragma unroll
     for (int i = 0; i < 9; i++) {
         const uint32_t r = kCX[i] == 1 ? xm : (kCX[i] == -1 ? xp : x);
         const uint32_t c = kCY[i] == 1 ? ym : (kCY[i] == -1 ? yp : y);
-        out[i] = Io<S>::ld(at32(src, (r * 9 + i) * ly + c));
+        out[i] = Io<S>::ld(at32(src, (r * 9 + i) * pitch + c));
     }
 }
 
@@ -382,49 +388,61 @@ struct FusedParams {
     int nt;
 };
 
+// Waves of 64 threads tile each row from y = 0 (the last wave of a row is
+// partly idle), so that with the line-aligned pitch every plane access of a
+// wave starts on a 64-element boundary: a row length that is not a multiple
+// of 32 nodes otherwise shifts most waves off the 128-byte lines (measured
+// 5-7 % slower at ly = 1800).  Wave w of the launch is row w / wpr, columns
+// 64 (w mod wpr) ... + 63.
 template <class S, int V, bool kProbe, bool kReduce>
 __global__ __launch_bounds__(kBlock) void k_fused(FusedParams<S> p) {
     using C = typename Io<S>::C;
     const uint32_t blk = p.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const Geom &g = p.g;
+    const uint32_t lane = threadIdx.x & 63;
     C v[V][9];
-    uint32_t node[V], xs[V], ys[V];
+    uint32_t xs[V], ys[V], ws[V];
     bool live[V];
 #pragma unroll
     for (int k = 0; k < V; k++) {
-        node[k] = (blk * V + k) * kBlock + threadIdx.x;
-        live[k] = node[k] < g.N;
-        xs[k] = ys[k] = 0;
-        if (live[k]) {
-            xs[k] = row_of(g, node[k]);
-            ys[k] = node[k] - xs[k] * g.ly;
-            pull<S>(p.src, g, xs[k], ys[k], v[k]);
-        }
+        const uint32_t w = __builtin_amdgcn_readfirstlane((blk * V + k) * (kBlock / 64) + (threadIdx.x >> 6));
+        const uint32_t x = g.wpr == 1 ? w : (uint32_t)__umul64hi(g.wpr_magic, (uint64_t)w);
+        ws[k] = w;
+        xs[k] = x;
+        ys[k] = (w - x * g.wpr) * 64 + lane;
+        live[k] = x < g.lx && ys[k] < g.ly;
+        if (live[k]) pull<S>(p.src, g, xs[k], ys[k], v[k]);
     }
     double acc0 = 0, acc1 = 0, acc2 = 0;
 #pragma unroll
     for (int k = 0; k < V; k++) {
-        if (!live[k]) continue;
-        uint32_t flags = 0;
-        if (p.bc.chunk) {
-            const uint32_t chunk = __builtin_amdgcn_readfirstlane(node[k] >> 6);
-            if (p.bc.chunk[chunk])
-                flags = apply_class<S>(p.bc, p.src, p.side_prev, g, node[k], xs[k], ys[k], v[k]);
-        }
-        NodeOut<S> o;
-        collide_node<S>(v[k], p.omega, o);
-        store_node<S>(p.dst, p.side_next, g, p.bc, xs[k], ys[k], flags, v[k], o, p.nt != 0);
-        if constexpr (kProbe) {
-            if ((int64_t)node[k] == p.probe_cell) {
-                p.probe_out[0] = o.rho;
-                p.probe_out[1] = (double)o.ux;
-                p.probe_out[2] = (double)o.uy;
+        if (live[k]) {
+            const uint32_t n = xs[k] * g.ly + ys[k];
+            uint32_t flags = 0;
+            if (p.bc.wave && p.bc.wave[ws[k]])  // wave w is row x, columns 64 (w mod wpr) on: its flag is entry w
+                flags = apply_class<S>(p.bc, p.src, p.side_prev, g, n, xs[k], ys[k], v[k]);
+            NodeOut<S> o;
+            collide_node<S>(v[k], p.omega, o);
+            store_node<S>(p.dst, p.side_next, g, p.bc, xs[k], ys[k], flags, v[k], o, p.nt != 0);
+            if constexpr (kProbe) {
+                if ((int64_t)n == p.probe_cell) {
+                    p.probe_out[0] = o.rho;
+                    p.probe_out[1] = (double)o.ux;
+                    p.probe_out[2] = (double)o.uy;
+                }
             }
-        }
-        if constexpr (kReduce) {
-            acc0 += o.rho;
-            acc1 += o.jx;
-            acc2 += o.jy;
+            if constexpr (kReduce) {
+                acc0 += o.rho;
+                acc1 += o.jx;
+                acc2 += o.jy;
+            }
+        } else if (xs[k] < g.lx && ys[k] < g.pitch) {
+            // lanes over the row padding write zeros there (never read), right
+            // after the wave's own stores, so that the last line of each plane
+            // row is written whole: a partly written line that leaves the
+            // cache costs the memory a read-modify-write
+#pragma unroll
+            for (int i = 0; i < 9; i++) Io<S>::st(p.dst + pop(g, xs[k], i, ys[k]), (C)0, p.nt != 0);
         }
     }
     if constexpr (kReduce) block_sum3(acc0, acc1, acc2, p.partials + 3 * (size_t)blockIdx.x);
@@ -747,7 +765,7 @@ __host__ __device__ inline SweepLds sweep_lds_layout(uint32_t ly, uint32_t nclas
 // load the node's own opposite population (the reflection's source) in place
 // of the upstream one, so that no load is needed later.
 template <class S>
-__device__ __forceinline__ void pull32_bc(const S *__restrict__ src, uint32_t ly, uint32_t xm, uint32_t x,
+__device__ __forceinline__ void pull32_bc(const S *__restrict__ src, uint32_t pitch, uint32_t xm, uint32_t x,
                                           uint32_t xp, uint32_t y, uint32_t ym, uint32_t yp, uint32_t w,
                                           typename Io<S>::C out[9]) {
 #pragma unroll
@@ -755,7 +773,7 @@ __device__ __forceinline__ void pull32_bc(const S *__restrict__ src, uint32_t ly
         const uint32_t r = kCX[i] == 1 ? xm : (kCX[i] == -1 ? xp : x);
         const uint32_t c = kCY[i] == 1 ? ym : (kCY[i] == -1 ? yp : y);
         const bool refl = ((w >> (2 * i)) & 3u) == kSrcReflect;
-        const uint32_t off = refl ? (x * 9 + kOpp[i]) * ly + y : (r * 9 + i) * ly + c;
+        const uint32_t off = refl ? (x * 9 + kOpp[i]) * pitch + y : (r * 9 + i) * pitch + c;
         out[i] = Io<S>::ld(at32(src, off));
     }
 }
@@ -771,7 +789,7 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
     using C = typename Io<S>::C;
     extern __shared__ __align__(16) unsigned char smem[];
     const Geom &g = p.g;
-    const uint32_t ly = g.ly, lx = g.lx;
+    const uint32_t ly = g.ly, lx = g.lx, pitch = g.pitch;
     const uint32_t T = blockDim.x;
     const uint32_t R = min(p.rows, lx - xa);
     const uint32_t nw = (ly + 63) / 64;
@@ -798,9 +816,7 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
             for (uint32_t i = threadIdx.x; i < 3 * ly; i += T) side_in[i] = p.side_prev[i];
         for (uint32_t i = threadIdx.x; i < (R + 2) * nw; i += T) {
             const uint32_t rr = i / nw, j = i - rr * nw;
-            const uint32_t n0 = row_at(rr) * ly + 64 * j;
-            const uint32_t n1 = min(n0 + 63, row_at(rr) * ly + ly - 1);
-            wflag[i] = p.bc.chunk[n0 >> 6] | p.bc.chunk[n1 >> 6];
+            wflag[i] = p.bc.wave[row_at(rr) * nw + j];
         }
         __syncthreads();
     }
@@ -832,9 +848,9 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
         for (int k = 0; k < V; k++) {
             if (special(rr, k)) {
                 const uint32_t w = code[k] ? cls[code[k]].src : 0u;
-                pull32_bc<S>(p.src, ly, xm, xr, xp, ys[k], ym[k], yp[k], w, out[k]);
+                pull32_bc<S>(p.src, pitch, xm, xr, xp, ys[k], ym[k], yp[k], w, out[k]);
             } else {
-                pull32<S>(p.src, ly, xm, xr, xp, ys[k], ym[k], yp[k], out[k]);
+                pull32<S>(p.src, pitch, xm, xr, xp, ys[k], ym[k], yp[k], out[k]);
             }
         }
     };
@@ -973,9 +989,12 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
                 NodeOut<S> o2;
                 collide_node<S>(v2, p.omega, o2);
                 // no pressure fix here, so every direction is stored and the
-                // equilibrium is not kept for a fix-up
+                // equilibrium is not kept for a fix-up; lanes past column ly-1
+                // store its values into the row padding (never read) so that
+                // every 128-byte line is written whole
+                const uint32_t yst = min(threadIdx.x + k * T, pitch - 1);
 #pragma unroll
-                for (int q = 0; q < 9; q++) Io<S>::st(at32(p.dst, (r * 9 + q) * ly + y), o2.s[q], p.nt != 0);
+                for (int q = 0; q < 9; q++) Io<S>::st(at32(p.dst, (r * 9 + q) * pitch + yst), o2.s[q], p.nt != 0);
                 if (flags2 & kFlagOutletSide) {
                     p.side_next[(size_t)y * 3 + 0] = v2[3];
                     p.side_next[(size_t)y * 3 + 1] = v2[6];
@@ -1315,6 +1334,13 @@ __global__ void k_bc_pressure(Geom g, BcView bc, double *__restrict__ f, const d
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// Blocks of k_fused with V nodes per thread: lx * wpr waves, kBlock / 64 waves
+// per block and node slot.
+inline int64_t fused_grid(const Geom &g, int V) {
+    const int64_t per = (int64_t)(kBlock / 64) * V;
+    return ((int64_t)g.lx * g.wpr + per - 1) / per;
+}
+
 template <class S, int V>
 hipError_t fused_dispatch(const FusedParams<S> &p, unsigned grid, bool probe, bool reduce, hipStream_t s) {
     if (probe && reduce) hipLaunchKernelGGL((k_fused<S, V, true, true>), dim3(grid), dim3(kBlock), 0, s, p);
@@ -1356,7 +1382,7 @@ hipError_t fused_typed(const Geom &g, const StateView &st, int src, const BcView
         else hipLaunchKernelGGL((k_fused16<S, false, false>), dim3(grid), dim3(kBlock), 0, s, p);
         return hipGetLastError();
     }
-    const unsigned grid = (unsigned)(((int64_t)g.N + (int64_t)kBlock * V - 1) / ((int64_t)kBlock * V));
+    const unsigned grid = (unsigned)fused_grid(g, V);
     if (nblocks) *nblocks = (int)grid;
     switch (V) {
         case 1: return fused_dispatch<S, 1>(p, grid, probe, partials != nullptr, s);
@@ -1397,8 +1423,8 @@ hipError_t launch_fused_batch(int storage, int v, const BatchJob *jobs, const ui
 int vec16_nodes(int storage) { return storage == LBM_STORE_F64 ? 2 : (storage == LBM_STORE_F32 ? 4 : 8); }
 
 int fused_blocks(const Geom &g, const LaunchCfg &cfg) {
-    const int64_t per = (int64_t)kBlock * (cfg.cells_per_thread ? cfg.cells_per_thread : 1);
-    return (int)(((int64_t)g.N + per - 1) / per);
+    // 16-byte groups (0) need fewer blocks than one node per thread
+    return (int)fused_grid(g, cfg.cells_per_thread ? cfg.cells_per_thread : 1);
 }
 
 int partial_rows_needed(const Geom &g, const LaunchCfg &cfg) {
@@ -1502,8 +1528,9 @@ bool sweep_supported(int storage, const Geom &g, const BcView &bc) {
     // the 128 registers of a 1024-thread block without spilling
     const uint32_t max_ly = storage == LBM_STORE_F64 ? 1024 : 2048;
     if (g.lx < 3 || g.ly < 1 || g.ly > max_ly) return false;
-    // 9 N sizeof(S) bytes addressed with 32-bit offsets
-    if ((uint64_t)g.N * 9 * (storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2)) >= (1ull << 32))
+    // 9 lx pitch sizeof(S) bytes addressed with 32-bit offsets
+    if ((uint64_t)g.lx * g.pitch * 9 * (storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2)) >=
+        (1ull << 32))
         return false;
     return sweep_lds_for(storage, g.ly, bc.chunk ? bc.nclasses : 0, kSweepMaxRows) <= kLdsLimit;
 }

@@ -12,6 +12,7 @@ namespace lbm {
 struct BcView {
     const uint16_t *codes = nullptr;       // class per node, N
     const uint8_t *chunk = nullptr;        // nonzero if any node of a 64-node chunk is special
+    const uint8_t *wave = nullptr;         // nonzero if any node of wave (x, y/64) is special, x*wpr + y/64
     const ClassEntry *classes = nullptr;
     uint32_t nclasses = 0;                 // entries of classes (0 with no program)
     const uint32_t *special = nullptr;     // list of special nodes
@@ -21,13 +22,20 @@ struct BcView {
     uint32_t flags = 0;                    // kProg*
 };
 
-// Geometry with a precomputed divisor for node -> (x, y).
+// Geometry with a precomputed divisor for node -> (x, y).  Node n = x*ly + y
+// is the logical index (boundary codes, chunk flags, AoS host arrays); a
+// population plane row starts every `pitch` elements, ly rounded up to whole
+// 128-byte lines so that every row of every plane starts on a line.
 struct Geom {
     uint32_t lx = 0, ly = 0, N = 0;
-    uint64_t ly_magic = 0;  // floor(2^64 / ly) + 1 (exact 32-bit division by ly)
+    uint32_t pitch = 0;
+    uint32_t wpr = 0;        // 64-node waves per row, ceil(ly / 64)
+    uint64_t ly_magic = 0;   // floor(2^64 / ly) + 1 (exact 32-bit division by ly)
+    uint64_t wpr_magic = 0;  // the same for wpr
 };
 
-Geom make_geom(uint32_t lx, uint32_t ly);
+// `align`: elements per 128-byte line of the plane type (1 for AoS-only uses).
+Geom make_geom(uint32_t lx, uint32_t ly, uint32_t align = 1);
 
 // The state buffers of one lattice: nine planes (of the storage type) and the
 // outlet side buffers (3*ly of the compute type), each double-buffered.

@@ -59,11 +59,12 @@ class JobSpec:
         return self.omega
 
     def footprint(self) -> int:
-        """Device bytes of the job's context: two state buffers, FP64 staging,
-        boundary codes, probe ring."""
+        """Device bytes of the job's context: two state buffers (rows padded to
+        128-byte lines), FP64 staging, boundary codes, probe ring."""
         s = STORAGE_BYTES[self.storage]
+        pitch = -(-self.ly * s // 128) * 128 // s
         probe = 24 * self.steps if self.kind == "von_karman" else 0
-        return self.nodes * (18 * s + 96 + 2) + probe + (1 << 20)
+        return 18 * s * self.lx * pitch + self.nodes * (96 + 2) + probe + (1 << 20)
 
     def boundary(self):
         if self.kind == "von_karman":
