@@ -831,10 +831,9 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     // one-step jobs: tables for the two steps of a pair
     std::vector<BatchJob> fj[2];
     std::vector<uint32_t> fblocks;
-    const int per_block = batch_block_nodes(b->v);
     for (size_t m = 0; m < fjobs.size(); m++) {
         lbm_ctx *c = b->ctxs[fjobs[m]];
-        const uint32_t nb = (uint32_t)(((int64_t)c->g.N + per_block - 1) / per_block);
+        const uint32_t nb = (uint32_t)batch_job_blocks(c->g, b->v);
         const uint32_t block0 = (uint32_t)fblocks.size();
         fblocks.insert(fblocks.end(), nb, (uint32_t)m);
         for (int p = 0; p < 2; p++) {
@@ -1091,11 +1090,10 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
     // job tables for the two step parities of this run
     std::vector<BatchJob> jobs[2];
     std::vector<uint32_t> block_job;
-    const int per_block = batch_block_nodes(b->v);
     for (int p = 0; p < 2; p++) jobs[p].resize(n);
     for (int j = 0; j < n; j++) {
         lbm_ctx *c = b->ctxs[j];
-        const uint32_t nb = (uint32_t)(((int64_t)c->g.N + per_block - 1) / per_block);
+        const uint32_t nb = (uint32_t)batch_job_blocks(c->g, b->v);
         const uint32_t block0 = (uint32_t)block_job.size();
         block_job.insert(block_job.end(), nb, (uint32_t)j);
         for (int p = 0; p < 2; p++) {

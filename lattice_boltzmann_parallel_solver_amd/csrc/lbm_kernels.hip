@@ -641,16 +641,16 @@ __global__ __launch_bounds__(kBlock) void k_fused_batch(const BatchJob *__restri
     const uint32_t lb = blockIdx.x - d.block0;
 #pragma unroll
     for (int k = 0; k < V; k++) {
-        const uint32_t n = (lb * V + k) * kBlock + threadIdx.x;
-        if (n < g.N) {
-            const uint32_t x = row_of(g, n), y = n - x * g.ly;
+        // waves tile each row from y = 0, as in k_fused
+        const uint32_t w = __builtin_amdgcn_readfirstlane((lb * V + k) * (kBlock / 64) + (threadIdx.x >> 6));
+        const uint32_t x = g.wpr == 1 ? w : (uint32_t)__umul64hi(g.wpr_magic, (uint64_t)w);
+        const uint32_t y = (w - x * g.wpr) * 64 + (threadIdx.x & 63);
+        if (x < g.lx && y < g.ly) {
+            const uint32_t n = x * g.ly + y;
             C v[9];
             pull<S>(src, g, x, y, v);
             uint32_t flags = 0;
-            if (d.bc.chunk) {
-                const uint32_t chunk = __builtin_amdgcn_readfirstlane(n >> 6);
-                if (d.bc.chunk[chunk]) flags = apply_class<S>(d.bc, src, side_prev, g, n, x, y, v);
-            }
+            if (d.bc.wave && d.bc.wave[w]) flags = apply_class<S>(d.bc, src, side_prev, g, n, x, y, v);
             NodeOut<S> o;
             collide_node<S>(v, d.omega, o);
             store_node<S>(dst, side_next, g, d.bc, x, y, flags, v, o, false);
@@ -663,6 +663,10 @@ __global__ __launch_bounds__(kBlock) void k_fused_batch(const BatchJob *__restri
                     out[2] = (double)o.uy;
                 }
             }
+        } else if (x < g.lx && y < g.pitch) {
+            // complete the last line of each plane row (the pad is never read)
+#pragma unroll
+            for (int i = 0; i < 9; i++) Io<S>::st(dst + pop(g, x, i, y), (C)0, false);
         }
     }
 }
@@ -1394,7 +1398,7 @@ hipError_t fused_typed(const Geom &g, const StateView &st, int src, const BcView
 
 }  // namespace
 
-int batch_block_nodes(int nodes_per_thread) { return kBlock * nodes_per_thread; }
+int64_t batch_job_blocks(const Geom &g, int nodes_per_thread) { return fused_grid(g, nodes_per_thread); }
 
 hipError_t launch_fused_batch(int storage, int v, const BatchJob *jobs, const uint32_t *block_job, unsigned nblocks,
                               int64_t *step, unsigned *done, hipStream_t s) {

@@ -110,7 +110,9 @@ struct BatchJob {
 // followed by a one-thread kernel that advances the batch step counter *step.
 hipError_t launch_fused_batch(int storage, int nodes_per_thread, const BatchJob *jobs, const uint32_t *block_job,
                               unsigned nblocks, int64_t *step, unsigned *done, hipStream_t s);
-int batch_block_nodes(int nodes_per_thread);
+// Blocks of one job in the batched one-step launch (its rows' waves, kBlock/64
+// waves and nodes_per_thread nodes a thread).
+int64_t batch_job_blocks(const Geom &g, int nodes_per_thread);
 
 // One lattice of a batch advanced by the two-step sweep kernel, buffers
 // resolved for the pair parity of the launch (src -> dst).
