@@ -763,7 +763,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     std::vector<int> fjobs;  // one-step jobs
     for (int j = 0; j < n; j++) {
         lbm_ctx *c = b->ctxs[j];
-        if (!c->st.scratch || !sweep_supported(c->storage, c->g, c->bc.view)) {
+        if (!c->st.scratch || !sweep_batch_supported(c->storage, c->g, c->bc.view)) {
             fjobs.push_back(j);
             continue;
         }
@@ -1082,7 +1082,7 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
         int64_t all = 0, other = 0;
         for (lbm_ctx *c : b->ctxs) {
             all += c->g.N;
-            if (!c->st.scratch || !sweep_supported(c->storage, c->g, c->bc.view)) other += c->g.N;
+            if (!c->st.scratch || !sweep_batch_supported(c->storage, c->g, c->bc.view)) other += c->g.N;
         }
         if (other * 100 <= all * 15) return batch_run_sweep(b, nsteps);
     }

@@ -716,6 +716,9 @@ struct SweepParams {
     double *probe1;  // sample of step 1 (3 doubles) or null
     double *probe2;  // sample of step 2
     uint32_t rows;   // output rows per block
+    uint32_t ntiles; // column tiles per row range (1: whole rows)
+    uint32_t tw;     // output columns per tile (ly with whole rows)
+    uint32_t lw;     // LDS columns of a tile: tw + 2 with tiles (halo column each side), else ly
     uint32_t nchunks;
     int nt;
     int xcd;         // XCD-aware block order (neighbouring row ranges share an L2)
@@ -745,23 +748,24 @@ constexpr uint32_t kSweepMaxRows = 128;
 
 __host__ __device__ constexpr size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
 
-// Dynamic LDS of the sweep kernel: the y-exchange rows (two buffers of six
-// planes), the step-1 outlet values, the incoming outlet values (the input
-// side buffer), the class table and one "special" byte per (row, wave column)
-// of the block's rows.
+// Dynamic LDS of the sweep kernel for `cols` columns (ly, or a column tile
+// with its two halo columns): the y-exchange rows (two buffers of six planes),
+// the step-1 outlet values, the incoming outlet values (the input side
+// buffer), the class table and one "special" byte per (row, wave column) of
+// the block's rows.
 struct SweepLds {
     size_t side, side_in, cls, flag, total;
 };
 
 template <class S>
-__host__ __device__ inline SweepLds sweep_lds_layout(uint32_t ly, uint32_t nclasses, uint32_t rows) {
+__host__ __device__ inline SweepLds sweep_lds_layout(uint32_t cols, uint32_t nclasses, uint32_t rows) {
     using C = typename Io<S>::C;
     SweepLds L;
-    L.side = align16(12 * (size_t)ly * sizeof(S));
-    L.side_in = align16(L.side + 3 * (size_t)ly * sizeof(C));
-    L.cls = align16(L.side_in + 3 * (size_t)ly * sizeof(C));
+    L.side = align16(12 * (size_t)cols * sizeof(S));
+    L.side_in = align16(L.side + 3 * (size_t)cols * sizeof(C));
+    L.cls = align16(L.side_in + 3 * (size_t)cols * sizeof(C));
     L.flag = align16(L.cls + (size_t)nclasses * sizeof(ClassEntry));
-    L.total = L.flag + (size_t)(rows + 2) * ((ly + 63) / 64);
+    L.total = L.flag + (size_t)(rows + 2) * ((cols + 63) / 64);
     return L;
 }
 
@@ -788,17 +792,31 @@ struct Flag {
 };
 
 // The rows [xa, xa + p.rows) of one lattice (clipped at lx), two steps.
-template <class S, int V, bool kProbe>
-__device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa) {
+// With kTiled the block owns only output columns [cy0, cy0 + W) of them, cy0 =
+// tile * tw: step 1 also runs on one halo column each side (periodic in y),
+// whose values step 2 needs; the halo columns' step-2 results are not kept.
+// LDS and lane columns are then local: local column l is global column
+// cy0 - 1 + l (mod ly).
+template <class S, int V, bool kProbe, bool kTiled>
+__device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa, uint32_t tile) {
     using C = typename Io<S>::C;
     extern __shared__ __align__(16) unsigned char smem[];
     const Geom &g = p.g;
     const uint32_t ly = g.ly, lx = g.lx, pitch = g.pitch;
     const uint32_t T = blockDim.x;
     const uint32_t R = min(p.rows, lx - xa);
-    const uint32_t nw = (ly + 63) / 64;
+    const uint32_t cy0 = kTiled ? tile * p.tw : 0;  // first output column
+    const uint32_t W = kTiled ? min(p.tw, ly - cy0) : ly;  // output columns
+    const uint32_t lw = kTiled ? W + 2 : ly;              // computed columns
+    const uint32_t cw = kTiled ? p.lw : ly;               // LDS row stride
+    const uint32_t nw = (lw + 63) / 64;
     const bool has_bc = p.bc.chunk != nullptr;
-    const SweepLds L = sweep_lds_layout<S>(ly, has_bc ? p.bc.nclasses : 0, p.rows);
+    const SweepLds L = sweep_lds_layout<S>(cw, has_bc ? p.bc.nclasses : 0, p.rows);
+    // global column of local column l < lw (tiles only; cy0 - 1 + l lies in [-1, ly])
+    auto gcol = [&](uint32_t l) {
+        const uint32_t t = cy0 + l + ly - 1;
+        return t >= ly ? (t - ly >= ly ? t - 2 * ly : t - ly) : t;
+    };
     S *ex = reinterpret_cast<S *>(smem);
     C *side = reinterpret_cast<C *>(smem + L.side);
     C *side_in = reinterpret_cast<C *>(smem + L.side_in);
@@ -816,11 +834,33 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
         // step 1 reads the outlet values of the previous launch from LDS: only
         // blocks whose step-1 rows include lx-1 (the last block, and the first
         // through the periodic row xa-1) need them
-        if ((p.bc.flags & kProgOutlet) && (xa == 0 || xa + R + 1 >= lx))
-            for (uint32_t i = threadIdx.x; i < 3 * ly; i += T) side_in[i] = p.side_prev[i];
+        if ((p.bc.flags & kProgOutlet) && (xa == 0 || xa + R + 1 >= lx)) {
+            for (uint32_t i = threadIdx.x; i < 3 * lw; i += T) {
+                if constexpr (kTiled) {
+                    const uint32_t l = i / 3;
+                    side_in[i] = p.side_prev[gcol(l) * 3 + (i - 3 * l)];
+                } else {
+                    side_in[i] = p.side_prev[i];
+                }
+            }
+        }
         for (uint32_t i = threadIdx.x; i < (R + 2) * nw; i += T) {
             const uint32_t rr = i / nw, j = i - rr * nw;
-            wflag[i] = p.bc.wave[row_at(rr) * nw + j];
+            const uint8_t *wr = p.bc.wave + (size_t)row_at(rr) * g.wpr;
+            if constexpr (kTiled) {
+                // local wave j spans global columns a .. b (unwrapped, a >= -1,
+                // b <= ly): the global waves of both ends, and of both row
+                // ends where the span wraps
+                const int a = (int)cy0 - 1 + 64 * (int)j;
+                const int b = (int)cy0 - 1 + (int)min(64 * j + 63, lw - 1);
+                const uint32_t aw = (uint32_t)(a < 0 ? a + (int)ly : (a >= (int)ly ? a - (int)ly : a));
+                const uint32_t bw = (uint32_t)(b < 0 ? b + (int)ly : (b >= (int)ly ? b - (int)ly : b));
+                uint8_t f = wr[aw / 64] | wr[bw / 64];
+                if (a < 0 || b >= (int)ly) f |= wr[0] | wr[(ly - 1) / 64];
+                wflag[i] = f;
+            } else {
+                wflag[i] = wr[j];
+            }
         }
         __syncthreads();
     }
@@ -829,14 +869,29 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
     // values to the same places).  With no lane-dependent branch around memory
     // operations the compiler can count the stores issued after the prefetch
     // and does not wait for them at the top of the loop.
-    uint32_t ys[V], ym[V], yp[V], jw[V];
+    // ys/ym/yp: global column and its y neighbours (HBM pulls, codes, outlet
+    // buffer); with tiles ls/lm/lp are the local ones (LDS)
+    uint32_t ys[V], ym[V], yp[V], jw[V], ls[V], lm[V], lp[V];
 #pragma unroll
     for (int k = 0; k < V; k++) {
-        ys[k] = min(threadIdx.x + k * T, ly - 1);
+        const uint32_t l = min(threadIdx.x + k * T, lw - 1);
+        ys[k] = kTiled ? gcol(l) : l;
         ym[k] = ys[k] == 0 ? ly - 1 : ys[k] - 1;
         yp[k] = ys[k] + 1 >= ly ? 0 : ys[k] + 1;
         jw[k] = min((threadIdx.x >> 6) + k * (T >> 6), nw - 1);
+        ls[k] = l;
+        lm[k] = l == 0 ? 0 : l - 1;
+        lp[k] = min(l + 1, lw - 1);
     }
+    auto lcol = [&](int k) { return kTiled ? ls[k] : ys[k]; };  // LDS column of the lane
+    auto lmin = [&](int k) { return kTiled ? lm[k] : ym[k]; };  // LDS column of its y - 1
+    auto lplu = [&](int k) { return kTiled ? lp[k] : yp[k]; };  // LDS column of its y + 1
+    // the lane's column is an output column of this block (tiles: not a halo
+    // or clamped lane)
+    auto owned = [&](int k) {
+        const uint32_t lu = threadIdx.x + k * T;
+        return !kTiled || (lu >= 1 && lu <= W);
+    };
     // the wave holding column k has boundary nodes in row iteration rr (wave-uniform)
     auto special = [&](uint32_t rr, int k) -> bool {
         if (!has_bc) return false;
@@ -894,8 +949,8 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
 #pragma unroll
             for (int k = 0; k < V; k++) c2[k] = it + 2 <= R + 1 ? load_code(it + 2, k) : 0u;
         }
-        S *exb = ex + (it & 1) * 6 * (size_t)ly;
-        const S *exp_ = ex + ((it + 1) & 1) * 6 * (size_t)ly;
+        S *exb = ex + (it & 1) * 6 * (size_t)cw;
+        const S *exp_ = ex + ((it + 1) & 1) * 6 * (size_t)cw;
         C s013[V][3];
         // step 1 on row x
 #pragma unroll
@@ -915,26 +970,26 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
                         if constexpr (Io<S>::kDev) v[k][d] = (C)(ce.k[d] - kWd[d]);
                         else v[k][d] = ce.k[d];
                     } else if (kind == kSrcOutlet) {
-                        v[k][d] = side_in[(size_t)y * 3 + outlet_slot(d)];
+                        v[k][d] = side_in[(size_t)lcol(k) * 3 + outlet_slot(d)];
                     }
                 }
             }
             NodeOut<S> o;
             collide_node<S>(v[k], p.omega, o);
             if (flags & kFlagOutletSide) {
-                side[(size_t)y * 3 + 0] = v[k][3];
-                side[(size_t)y * 3 + 1] = v[k][6];
-                side[(size_t)y * 3 + 2] = v[k][7];
+                side[(size_t)lcol(k) * 3 + 0] = v[k][3];
+                side[(size_t)lcol(k) * 3 + 1] = v[k][6];
+                side[(size_t)lcol(k) * 3 + 2] = v[k][7];
             }
             if constexpr (kProbe) {
-                if (p.probe1 && it >= 1 && it <= R && (int64_t)(x * ly + y) == p.probe_cell) {
+                if (p.probe1 && it >= 1 && it <= R && owned(k) && (int64_t)(x * ly + y) == p.probe_cell) {
                     p.probe1[0] = o.rho;
                     p.probe1[1] = (double)o.ux;
                     p.probe1[2] = (double)o.uy;
                 }
             }
 #pragma unroll
-            for (int j = 0; j < 6; j++) lds_put<S>(exb + j * ly + y, o.s[kYM[j]]);
+            for (int j = 0; j < 6; j++) lds_put<S>(exb + j * cw + lcol(k), o.s[kYM[j]]);
             s013[k][0] = round_to_storage<S>(o.s[0]);
             s013[k][1] = round_to_storage<S>(o.s[1]);
             s013[k][2] = round_to_storage<S>(o.s[3]);
@@ -947,12 +1002,12 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
             Y[0] = s013[k][0];
             Y[1] = s013[k][1];
             Y[3] = s013[k][2];
-            Y[2] = lds_get<S>(exb + 0 * ly + ym[k]);
-            Y[5] = lds_get<S>(exb + 1 * ly + ym[k]);
-            Y[6] = lds_get<S>(exb + 2 * ly + ym[k]);
-            Y[4] = lds_get<S>(exb + 3 * ly + yp[k]);
-            Y[7] = lds_get<S>(exb + 4 * ly + yp[k]);
-            Y[8] = lds_get<S>(exb + 5 * ly + yp[k]);
+            Y[2] = lds_get<S>(exb + 0 * cw + lmin(k));
+            Y[5] = lds_get<S>(exb + 1 * cw + lmin(k));
+            Y[6] = lds_get<S>(exb + 2 * cw + lmin(k));
+            Y[4] = lds_get<S>(exb + 3 * cw + lplu(k));
+            Y[7] = lds_get<S>(exb + 4 * cw + lplu(k));
+            Y[8] = lds_get<S>(exb + 5 * cw + lplu(k));
             if constexpr (kS2) {
                 // step 2 on row r = x - 1
                 const uint32_t r = row_at(it - 1);
@@ -980,13 +1035,13 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
                         if (kind == kSrcReflect) {
                             const int s = kOpp[d];
                             const C own = s == 0 ? B[k][0] : s == 1 ? Cc[k][0] : s == 3 ? D[k]
-                                                                                   : lds_get<S>(exp_ + kYSlot[s] * ly + y);
+                                                                                   : lds_get<S>(exp_ + kYSlot[s] * cw + lcol(k));
                             v2[d] = own - (C)ce.k[d];
                         } else if (kind == kSrcConst) {
                             if constexpr (Io<S>::kDev) v2[d] = (C)(ce.k[d] - kWd[d]);
                             else v2[d] = ce.k[d];
                         } else if (kind == kSrcOutlet) {
-                            v2[d] = side[(size_t)y * 3 + outlet_slot(d)];
+                            v2[d] = side[(size_t)lcol(k) * 3 + outlet_slot(d)];
                         }
                     }
                 }
@@ -996,16 +1051,29 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
                 // equilibrium is not kept for a fix-up; lanes past column ly-1
                 // store its values into the row padding (never read) so that
                 // every 128-byte line is written whole
-                const uint32_t yst = min(threadIdx.x + k * T, pitch - 1);
+                if constexpr (kTiled) {
+                    // output and padding columns go to the output; halo lanes,
+                    // and clamped lanes over other tiles' columns, to scratch
+                    const uint32_t lu = threadIdx.x + k * T;
+                    const uint32_t vc = cy0 + lu - 1;  // lu = 0 only with the left halo
+                    const bool keep = lu >= 1 && (lu <= W || (vc >= ly && vc < pitch));
+                    S *base = keep ? p.dst + (size_t)r * 9 * pitch + vc : p.scratch + (size_t)k * 9 * T + threadIdx.x;
+                    const uint32_t stride = keep ? pitch : T;
 #pragma unroll
-                for (int q = 0; q < 9; q++) Io<S>::st(at32(p.dst, (r * 9 + q) * pitch + yst), o2.s[q], p.nt != 0);
-                if (flags2 & kFlagOutletSide) {
+                    for (int q = 0; q < 9; q++) Io<S>::st(base + (size_t)q * stride, o2.s[q], p.nt != 0);
+                } else {
+                    const uint32_t yst = min(threadIdx.x + k * T, pitch - 1);
+#pragma unroll
+                    for (int q = 0; q < 9; q++)
+                        Io<S>::st(at32(p.dst, (r * 9 + q) * pitch + yst), o2.s[q], p.nt != 0);
+                }
+                if ((flags2 & kFlagOutletSide) && owned(k)) {
                     p.side_next[(size_t)y * 3 + 0] = v2[3];
                     p.side_next[(size_t)y * 3 + 1] = v2[6];
                     p.side_next[(size_t)y * 3 + 2] = v2[7];
                 }
                 if constexpr (kProbe) {
-                    if (p.probe2 && (int64_t)(r * ly + y) == p.probe_cell) {
+                    if (p.probe2 && owned(k) && (int64_t)(r * ly + y) == p.probe_cell) {
                         p.probe2[0] = o2.rho;
                         p.probe2[1] = (double)o2.ux;
                         p.probe2[2] = (double)o2.uy;
@@ -1042,10 +1110,16 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
     body(R + 1, Flag<false>{}, Flag<true>{});
 }
 
-template <class S, int V, bool kProbe>
+template <class S, int V, bool kProbe, bool kTiled>
 __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
     const uint32_t blk = p.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    sweep_block<S, V, kProbe>(p, blk * p.rows);
+    if constexpr (kTiled) {
+        // the tiles of one row range on consecutive blocks (one XCD's L2)
+        const uint32_t rb = blk / p.ntiles;
+        sweep_block<S, V, kProbe, true>(p, rb * p.rows, blk - rb * p.ntiles);
+    } else {
+        sweep_block<S, V, kProbe, false>(p, blk * p.rows, 0);
+    }
 }
 
 // Two steps of every job of a batch bucket: block b sweeps rows of job
@@ -1079,7 +1153,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2_batch(const SweepJob *
     p.nt = 0;
     p.xcd = 1;
     p.scratch = static_cast<S *>(j.scratch);
-    sweep_block<S, V, true>(p, (blk - j.block0) * j.rows);
+    p.ntiles = 1;
+    p.tw = p.lw = j.g.ly;
+    sweep_block<S, V, true, false>(p, (blk - j.block0) * j.rows, 0);
 }
 
 __global__ void k_add_step(int64_t *step, int64_t k) { *step += k; }
@@ -1460,9 +1536,9 @@ size_t sweep_lds_for(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows)
     }
 }
 
-template <class S, int V, bool kProbe>
+template <class S, int V, bool kProbe, bool kTiled>
 hipError_t sweep_go(const SweepParams<S> &p, unsigned grid, unsigned threads, size_t lds, hipStream_t s) {
-    auto fn = k_sweep2<S, V, kProbe>;
+    auto fn = k_sweep2<S, V, kProbe, kTiled>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;
@@ -1490,29 +1566,43 @@ hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView
     p.nt = cfg.nt_stores;
     p.xcd = cfg.xcd_remap;
     p.scratch = static_cast<S *>(st.scratch);
-    const int V = g.ly <= 1024 ? 1 : 2;
-    const uint32_t per = (g.ly + V - 1) / V;
-    const unsigned threads = (per + 63) / 64 * 64;
+    const SweepShape sh = sweep_shape(sizeof(S) == 8 ? LBM_STORE_F64 : (sizeof(S) == 4 ? LBM_STORE_F32 : LBM_STORE_F16),
+                                      g.ly);
+    const int V = sh.v;
+    const unsigned threads = sh.threads;
+    const bool tiled = sh.ntiles > 1;
+    p.ntiles = sh.ntiles;
+    p.tw = sh.tw;
+    p.lw = sh.lw;
     if (!p.scratch || (size_t)9 * V * threads * sizeof(S) > kSweepScratchBytes) return hipErrorInvalidValue;
     int slots = 0;
     if (cfg.sweep_rows <= 0) {
-        // resident blocks per CU (LDS and registers) times the CU count
+        // resident blocks per CU (LDS and registers) times the CU count, in
+        // row ranges (the tiles of a range are separate blocks)
         int dev = 0, cus = 0, per_cu = 0;
-        const size_t lds_est = sweep_lds_layout<S>(g.ly, bc.chunk ? bc.nclasses : 0, 64).total;
-        const void *fn = V == 1 ? reinterpret_cast<const void *>(k_sweep2<S, 1, false>)
-                                : reinterpret_cast<const void *>(k_sweep2<S, sizeof(S) == 8 ? 1 : 2, false>);
+        const size_t lds_est = sweep_lds_layout<S>(sh.lw, bc.chunk ? bc.nclasses : 0, 64).total;
+        constexpr int V2 = sizeof(S) == 8 ? 1 : 2;
+        const void *fn = V == 1 ? (tiled ? reinterpret_cast<const void *>(k_sweep2<S, 1, false, true>)
+                                         : reinterpret_cast<const void *>(k_sweep2<S, 1, false, false>))
+                                : (tiled ? reinterpret_cast<const void *>(k_sweep2<S, V2, false, true>)
+                                         : reinterpret_cast<const void *>(k_sweep2<S, V2, false, false>));
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
             hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_est) == hipSuccess &&
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds_est) == hipSuccess)
-            slots = cus * (per_cu > 0 ? per_cu : 1);
+            slots = std::max(1, cus * (per_cu > 0 ? per_cu : 1) / (int)sh.ntiles);
     }
     p.rows = (uint32_t)sweep_rows_for(g, cfg, slots);
-    const unsigned grid = (g.lx + p.rows - 1) / p.rows;
-    const size_t lds = sweep_lds_layout<S>(g.ly, bc.chunk ? bc.nclasses : 0, p.rows).total;
+    const unsigned grid = (g.lx + p.rows - 1) / p.rows * sh.ntiles;
+    const size_t lds = sweep_lds_layout<S>(sh.lw, bc.chunk ? bc.nclasses : 0, p.rows).total;
 #define LBM_SWEEP(VV)                                                                     \
-    return probing ? sweep_go<S, VV, true>(p, grid, threads, lds, s)                      \
-                   : sweep_go<S, VV, false>(p, grid, threads, lds, s)
+    do {                                                                                  \
+        if (tiled)                                                                        \
+            return probing ? sweep_go<S, VV, true, true>(p, grid, threads, lds, s)        \
+                           : sweep_go<S, VV, false, true>(p, grid, threads, lds, s);      \
+        return probing ? sweep_go<S, VV, true, false>(p, grid, threads, lds, s)           \
+                       : sweep_go<S, VV, false, false>(p, grid, threads, lds, s);         \
+    } while (0)
     if constexpr (sizeof(S) == 8) {
         if (V != 1) return hipErrorInvalidValue;
         LBM_SWEEP(1);
@@ -1525,18 +1615,42 @@ hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView
 
 }  // namespace
 
+SweepShape sweep_shape(int storage, uint32_t ly) {
+    // one column per thread for FP64, two for FP32/FP16: the shapes that fit
+    // the 128 registers of a 1024-thread block without spilling
+    const uint32_t vmax = storage == LBM_STORE_F64 ? 1 : 2;
+    const uint32_t maxc = (uint32_t)kSweepThreads * vmax;
+    SweepShape sh;
+    if (ly <= maxc) {
+        sh.ntiles = 1;
+        sh.tw = sh.lw = ly;
+    } else {
+        // the fewest tiles whose output columns plus two halo columns fit a block
+        sh.ntiles = (ly + maxc - 3) / (maxc - 2);
+        sh.tw = (ly + sh.ntiles - 1) / sh.ntiles;
+        sh.lw = sh.tw + 2;
+    }
+    sh.v = (storage == LBM_STORE_F64 || sh.lw <= (uint32_t)kSweepThreads) ? 1 : 2;
+    sh.threads = ((sh.lw + sh.v - 1) / sh.v + 63) / 64 * 64;
+    return sh;
+}
+
 bool sweep_supported(int storage, const Geom &g, const BcView &bc) {
     if (bc.flags & kProgPressure) return false;
     if (bc.chunk && bc.nclasses > kSweepMaxClasses) return false;
-    // one column per thread for FP64, two for FP32/FP16: the shapes that fit
-    // the 128 registers of a 1024-thread block without spilling
-    const uint32_t max_ly = storage == LBM_STORE_F64 ? 1024 : 2048;
-    if (g.lx < 3 || g.ly < 1 || g.ly > max_ly) return false;
+    if (g.lx < 3 || g.ly < 1) return false;
+    const SweepShape sh = sweep_shape(storage, g.ly);
+    // every tile holds at least one output column
+    if (sh.ntiles > 1 && (uint64_t)sh.tw * (sh.ntiles - 1) >= g.ly) return false;
     // 9 lx pitch sizeof(S) bytes addressed with 32-bit offsets
     if ((uint64_t)g.lx * g.pitch * 9 * (storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2)) >=
         (1ull << 32))
         return false;
-    return sweep_lds_for(storage, g.ly, bc.chunk ? bc.nclasses : 0, kSweepMaxRows) <= kLdsLimit;
+    return sweep_lds_for(storage, sh.lw, bc.chunk ? bc.nclasses : 0, kSweepMaxRows) <= kLdsLimit;
+}
+
+bool sweep_batch_supported(int storage, const Geom &g, const BcView &bc) {
+    return sweep_supported(storage, g, bc) && sweep_shape(storage, g.ly).ntiles == 1;
 }
 
 int sweep_rows_for(const Geom &g, const LaunchCfg &cfg, int slots) {

@@ -82,6 +82,16 @@ hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int sr
 // 32-bit byte offsets, and its LDS (12 ly storage values, 3 ly compute values,
 // the class table, one byte per row and wave column) within 160 KiB.
 bool sweep_supported(int storage, const Geom &g, const BcView &bc);
+// The batched sweep takes whole rows only (no column tiles).
+bool sweep_batch_supported(int storage, const Geom &g, const BcView &bc);
+// Launch shape of the sweep kernel for rows of ly nodes: columns per thread,
+// column tiles (1: whole rows; else tiles of tw output columns plus a halo
+// column each side, lw = tw + 2 computed columns), threads per block.
+struct SweepShape {
+    int v = 1;
+    uint32_t ntiles = 1, tw = 0, lw = 0, threads = 0;
+};
+SweepShape sweep_shape(int storage, uint32_t ly);
 // Output rows per block of the sweep kernel: cfg.sweep_rows if set, else
 // spread over whole waves of `slots` resident blocks (at least 8 rows).
 int sweep_rows_for(const Geom &g, const LaunchCfg &cfg, int slots);
