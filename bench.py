@@ -16,7 +16,9 @@ roofline   : HBM bound; achieved = algorithmic bytes per launch (every populatio
              single_step_equivalent_gbs: the bandwidth a one-step-per-launch kernel
              would need for the same MLUPS (72 B per node step).
 single_step: the same workload with one step per launch (k_fused).
-fp64, f16  : the same workload with FP64 populations / FP16 deviation storage.
+fp64, f16  : the same workload with FP64 populations / FP16 deviation storage,
+             two steps per launch (FP64 rows of 2048 split into three column
+             tiles); fp64_single_step: FP64 with one step per launch.
 latency    : p50/p99 time-to-result of the continuous-batching server on a seeded
              Poisson job stream (scaled serving mix, stated in the line).
 batching   : aggregate MLUPS of seeded uneven channels advanced by one launch per
@@ -119,12 +121,12 @@ def batching_leg(n=256, steps=1000, seed=0):
     """The 256 seeded uneven channels of the batching workload (nx U[200,1200],
     ny = 3nx/7, blockage U[0.1,0.4], Re U[40,200], FP32), 1000 steps: one
     batch (two steps per launch) versus the same channels solved one at a
-    time (one step per launch, their fastest form)."""
+    time, each at the autotuned launch shape of its size bucket."""
     from lattice_boltzmann_parallel_solver_amd import workloads as W
 
     jobs = W.channel_jobs(n, seed=seed)
     _, batched, _ = W.run_batched(jobs, steps, "f32", temporal=2)
-    _, sequential = W.run_sequential(jobs, steps, "f32")
+    _, sequential = W.run_sequential(jobs, steps, "f32", tune=True)
     return {"jobs": n, "steps": steps, "nodes": sum(j.nodes for j in jobs), "batched_mlups": batched,
             "sequential_mlups": sequential, "speedup": batched / sequential,
             "workload": f"{n} channels, seed {seed}, {steps} steps, f32; full-length run (20k steps) in "
@@ -257,6 +259,7 @@ def main():
 
     single = leg(args.storage, args.fp64_steps, 1) if spl == 2 else None
     fp64 = leg("f64", args.fp64_steps, args.temporal) if args.storage != "f64" else None
+    fp64_single = leg("f64", args.fp64_steps, 1) if args.storage != "f64" and args.temporal == 2 else None
     f16 = leg("f16", args.f16_steps, args.temporal) if args.storage != "f16" else None
 
     latency = None if args.no_serving else serving_leg()
@@ -287,6 +290,7 @@ def main():
                      "single_step_equivalent_gbs": mlups * 1e6 * BYTES_PER_NODE[args.storage] / 1e9},
         "single_step": single,
         "fp64": fp64,
+        "fp64_single_step": fp64_single,
         "f16": f16,
         "latency": latency,
         "batching": batching,
