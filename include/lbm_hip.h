@@ -162,10 +162,12 @@ int lbm_set_launch(lbm_ctx *ctx, int cells_per_thread, int xcd_remap, int nt_sto
  * kernel launch (a row sweep that keeps the intermediate state on chip and
  * reads and writes the populations once per two steps; results are identical
  * to single steps), 1 = one step per launch (default).  rows_per_block: rows
- * each block of the two-step kernel sweeps (0 = automatic).  Two steps need
- * no pressure fix in the boundary program, nx >= 3 and ny <= 2048 (FP32,
- * FP16) or 1024 (FP64); otherwise LBM_ERR_ARG.  No counterpart in the
- * reference (a launch setting of this build). */
+ * each block of the two-step kernel sweeps (0 = automatic).  Rows wider than
+ * one block (ny > 2048 FP32/FP16, > 1024 FP64) run as column tiles with a
+ * halo column each side.  Two steps need no pressure fix in the boundary
+ * program, nx >= 3, at most 128 boundary classes and 9*nx*pitch*sizeof
+ * (storage) < 2^32 bytes; otherwise LBM_ERR_ARG.  No counterpart in the
+ * reference (a launch setting of this library). */
 int lbm_set_temporal(lbm_ctx *ctx, int steps_per_launch, int rows_per_block);
 
 /* ---- batches: many lattices advanced by one launch per step ------------- */

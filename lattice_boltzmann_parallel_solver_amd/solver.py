@@ -158,8 +158,9 @@ class Lattice:
     def set_temporal(self, steps_per_launch: int = 2, rows_per_block: int = 0) -> None:
         """Advance `steps_per_launch` (1 or 2) steps per kernel launch; two steps
         keep the intermediate state on chip (identical results, half the
-        population traffic).  Raises LbmError where the two-step kernel does not
-        apply (pressure fix, ny too large)."""
+        population traffic).  Rows wider than one kernel block run as column
+        tiles.  Raises LbmError where the two-step kernel does not apply
+        (pressure fix, fewer than three rows, more than 128 boundary classes)."""
         check(_lib.load().lbm_set_temporal(self._h, int(steps_per_launch), int(rows_per_block)))
 
     # -- state ------------------------------------------------------------------
