@@ -113,7 +113,8 @@ def serving_leg(n=24, seed=0):
     recs = server.run(arrivals)
     s = summarize(recs, server.makespan)
     s["workload"] = (f"{n} jobs, Poisson 50/s, 60% von Karman 2048x512, 20% Couette, 20% Poiseuille "
-                     "squares U[128,1024], budgets log-uniform [5k,20k], f32, seed 0")
+                     "squares U[128,1024], budgets log-uniform [5k,20k], f32, seed 0, "
+                     f"policy {server.policy}")
     return s
 
 
