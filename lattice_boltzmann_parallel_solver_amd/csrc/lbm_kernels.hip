@@ -266,7 +266,16 @@ __device__ __forceinline__ void collide_node(const typename Io<S>::C v[9], doubl
         const float rho = 1.0f + drho;
         const float jx = ((v[1] - v[3]) + (v[5] - v[7])) + (v[8] - v[6]);
         const float jy = ((v[2] - v[4]) + (v[5] - v[7])) + (v[6] - v[8]);
-        const float inv = rho != 0.0f ? 1.0f / rho : 0.0f;
+        // FP16 storage: the hardware reciprocal (1 ulp) in place of a correctly
+        // rounded division, a tenth of the collision's instructions and far
+        // below the storage rounding (5-10 % faster).  FP32 keeps the division:
+        // the reciprocal doubled its error against FP64 (8192x2048 channel,
+        // 50k steps: |u| L2 2.2e-4 vs 8.4e-5) and gained no speed.
+        float inv = 0.0f;
+        if (rho != 0.0f) {
+            if constexpr (sizeof(S) == 2) inv = __builtin_amdgcn_rcpf(rho);
+            else inv = 1.0f / rho;
+        }
         const float ux = jx * inv, uy = jy * inv;
         const float base = -1.5f * __builtin_fmaf(ux, ux, uy * uy);
         const float cu[9] = {0.0f, ux, uy, -ux, -uy, ux + uy, uy - ux, -(ux + uy), ux - uy};
