@@ -758,7 +758,8 @@ constexpr uint32_t kSweepMaxRows = 128;
 __host__ __device__ constexpr size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
 
 // Dynamic LDS of the sweep kernel for `cols` columns (ly, or a column tile
-// with its two halo columns): the y-exchange rows (two buffers of six planes),
+// with its two halo columns): the y-exchange rows (two buffers of six planes
+// of cols + 1 values, the last a spare column for lanes past the last column),
 // the step-1 outlet values, the incoming outlet values (the input side
 // buffer), the class table and one "special" byte per (row, wave column) of
 // the block's rows.
@@ -770,7 +771,7 @@ template <class S>
 __host__ __device__ inline SweepLds sweep_lds_layout(uint32_t cols, uint32_t nclasses, uint32_t rows) {
     using C = typename Io<S>::C;
     SweepLds L;
-    L.side = align16(12 * (size_t)cols * sizeof(S));
+    L.side = align16(12 * ((size_t)cols + 1) * sizeof(S));  // two buffers of six planes of cols + 1
     L.side_in = align16(L.side + 3 * (size_t)cols * sizeof(C));
     L.cls = align16(L.side_in + 3 * (size_t)cols * sizeof(C));
     L.flag = align16(L.cls + (size_t)nclasses * sizeof(ClassEntry));
@@ -817,7 +818,8 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
     const uint32_t cy0 = kTiled ? tile * p.tw : 0;  // first output column
     const uint32_t W = kTiled ? min(p.tw, ly - cy0) : ly;  // output columns
     const uint32_t lw = kTiled ? W + 2 : ly;              // computed columns
-    const uint32_t cw = kTiled ? p.lw : ly;               // LDS row stride
+    const uint32_t cw = kTiled ? p.lw : ly;               // LDS columns
+    const uint32_t cs = cw + 1;                           // LDS plane stride (a spare column)
     const uint32_t nw = (lw + 63) / 64;
     const bool has_bc = p.bc.chunk != nullptr;
     const SweepLds L = sweep_lds_layout<S>(cw, has_bc ? p.bc.nclasses : 0, p.rows);
@@ -958,8 +960,8 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
 #pragma unroll
             for (int k = 0; k < V; k++) c2[k] = it + 2 <= R + 1 ? load_code(it + 2, k) : 0u;
         }
-        S *exb = ex + (it & 1) * 6 * (size_t)cw;
-        const S *exp_ = ex + ((it + 1) & 1) * 6 * (size_t)cw;
+        S *exb = ex + (it & 1) * 6 * (size_t)cs;
+        const S *exp_ = ex + ((it + 1) & 1) * 6 * (size_t)cs;
         C s013[V][3];
         // step 1 on row x
 #pragma unroll
@@ -997,8 +999,17 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
                     p.probe1[2] = (double)o.uy;
                 }
             }
+            // Only the column's own lane publishes it.  A lane past the last
+            // column repeats it, possibly in another wave, which may already be
+            // one iteration ahead: its write of this buffer would overwrite the
+            // values the owner still reads as its reflection sources.  Such
+            // lanes write the spare column at the end of each plane (or, in a
+            // narrower last tile, columns past its lw that nobody reads) instead.
+            {
+                const uint32_t col = min(threadIdx.x + k * T, cw);
 #pragma unroll
-            for (int j = 0; j < 6; j++) lds_put<S>(exb + j * cw + lcol(k), o.s[kYM[j]]);
+                for (int j = 0; j < 6; j++) lds_put<S>(exb + j * cs + col, o.s[kYM[j]]);
+            }
             s013[k][0] = round_to_storage<S>(o.s[0]);
             s013[k][1] = round_to_storage<S>(o.s[1]);
             s013[k][2] = round_to_storage<S>(o.s[3]);
@@ -1011,12 +1022,12 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
             Y[0] = s013[k][0];
             Y[1] = s013[k][1];
             Y[3] = s013[k][2];
-            Y[2] = lds_get<S>(exb + 0 * cw + lmin(k));
-            Y[5] = lds_get<S>(exb + 1 * cw + lmin(k));
-            Y[6] = lds_get<S>(exb + 2 * cw + lmin(k));
-            Y[4] = lds_get<S>(exb + 3 * cw + lplu(k));
-            Y[7] = lds_get<S>(exb + 4 * cw + lplu(k));
-            Y[8] = lds_get<S>(exb + 5 * cw + lplu(k));
+            Y[2] = lds_get<S>(exb + 0 * cs + lmin(k));
+            Y[5] = lds_get<S>(exb + 1 * cs + lmin(k));
+            Y[6] = lds_get<S>(exb + 2 * cs + lmin(k));
+            Y[4] = lds_get<S>(exb + 3 * cs + lplu(k));
+            Y[7] = lds_get<S>(exb + 4 * cs + lplu(k));
+            Y[8] = lds_get<S>(exb + 5 * cs + lplu(k));
             if constexpr (kS2) {
                 // step 2 on row r = x - 1
                 const uint32_t r = row_at(it - 1);
@@ -1044,7 +1055,7 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
                         if (kind == kSrcReflect) {
                             const int s = kOpp[d];
                             const C own = s == 0 ? B[k][0] : s == 1 ? Cc[k][0] : s == 3 ? D[k]
-                                                                                   : lds_get<S>(exp_ + kYSlot[s] * cw + lcol(k));
+                                                                                   : lds_get<S>(exp_ + kYSlot[s] * cs + lcol(k));
                             v2[d] = own - (C)ce.k[d];
                         } else if (kind == kSrcConst) {
                             if constexpr (Io<S>::kDev) v2[d] = (C)(ce.k[d] - kWd[d]);

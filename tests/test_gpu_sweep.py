@@ -77,6 +77,7 @@ CASES = [
     (31, 1024, "f64", "couette", 8, 0),
     (26, 200, "f32", "couette", 10, 6),
     (26, 200, "f16", "periodic", 10, 100),
+    (16, 1030, "f32", "channel", 9, 4),        # a whole wave of lanes past the last column (walls there)
     # column tiles: rows wider than one block, halo columns each side
     (20, 1100, "f64", "channel", 8, 3),        # two tiles of 550
     (9, 2100, "f64", "periodic", 6, 0),        # three tiles, periodic wrap at both row ends
