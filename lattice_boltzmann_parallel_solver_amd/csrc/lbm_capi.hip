@@ -767,8 +767,9 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             fjobs.push_back(j);
             continue;
         }
-        const int v = c->g.ly <= 1024 ? 1 : 2;
-        const unsigned threads = (unsigned)(((c->g.ly + v - 1) / v + 63) / 64 * 64);
+        const SweepShape sh = sweep_shape(c->storage, c->g.ly);  // whole rows here
+        const int v = sh.v;
+        const unsigned threads = sh.threads;
         Bucket *bk = nullptr;
         for (Bucket &x : buckets)
             if (x.v == v && x.threads == threads) bk = &x;
@@ -814,7 +815,6 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
                 d.bc = c->bc.view;
                 d.omega = b->omegas[bk.members[m]];
                 d.rows = rows;
-                d.nchunks = (uint32_t)(((uint64_t)c->g.N + 63) / 64);
                 d.block0 = block0;
                 if (c->probe.cell >= 0) {
                     d.probe_cell = c->probe.cell;

@@ -683,10 +683,11 @@ __global__ __launch_bounds__(kBlock) void k_fused_batch(const BatchJob *__restri
 __global__ void k_next_step(int64_t *step) { *step += 1; }
 
 // ---------------------------------------------------------------------------
-// two steps per launch: temporal blocking over whole rows
+// two steps per launch: temporal blocking over rows
 //
-// A block owns whole rows [xa, xb) of the output and all ly nodes of each row
-// (thread t handles columns t, t + T, ...).  It sweeps x upwards; iteration
+// A block owns rows [xa, xb) of the output and all ly nodes of each row, or,
+// for rows wider than a block, one column tile of them plus a halo column on
+// each side (thread t handles columns t, t + T, ...).  It sweeps x upwards; iteration
 // it = 0 .. R+1 (R = xb - xa)
 //   1. runs step 1 for row x = xa - 1 + it (periodic): pull from the input
 //      buffer, boundary class, collision; the relaxed populations s1 stay on
@@ -728,7 +729,6 @@ struct SweepParams {
     uint32_t ntiles; // column tiles per row range (1: whole rows)
     uint32_t tw;     // output columns per tile (ly with whole rows)
     uint32_t lw;     // LDS columns of a tile: tw + 2 with tiles (halo column each side), else ly
-    uint32_t nchunks;
     int nt;
     int xcd;         // XCD-aware block order (neighbouring row ranges share an L2)
     S *scratch;      // 9 V blockDim.x values, written and never read
@@ -1169,7 +1169,6 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2_batch(const SweepJob *
         if (slot + 1 >= 0) p.probe2 = j.probe_ring + 3 * ((slot + 1) % j.probe_cap);
     }
     p.rows = j.rows;
-    p.nchunks = j.nchunks;
     p.nt = 0;
     p.xcd = 1;
     p.scratch = static_cast<S *>(j.scratch);
@@ -1582,7 +1581,6 @@ hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView
     p.probe_cell = probing ? pr.cell : -1;
     p.probe1 = probing && slot1 >= 0 ? pr.ring + 3 * (slot1 % pr.capacity) : nullptr;
     p.probe2 = probing && slot2 >= 0 ? pr.ring + 3 * (slot2 % pr.capacity) : nullptr;
-    p.nchunks = (uint32_t)(((uint64_t)g.N + 63) / 64);
     p.nt = cfg.nt_stores;
     p.xcd = cfg.xcd_remap;
     p.scratch = static_cast<S *>(st.scratch);

@@ -77,10 +77,11 @@ int fused_blocks(const Geom &g, const LaunchCfg &cfg);
 hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int src, const BcView &bc,
                          double omega, const ProbeView &pr, int64_t probe_slot1, int64_t probe_slot2,
                          const LaunchCfg &cfg, hipStream_t s);
-// The sweep kernel needs no pressure fix, at most 256 boundary classes,
-// lx >= 3, ly <= 2048 (FP32, FP16) or 1024 (FP64), 9 N values addressable with
-// 32-bit byte offsets, and its LDS (12 ly storage values, 3 ly compute values,
-// the class table, one byte per row and wave column) within 160 KiB.
+// The sweep kernel needs no pressure fix, at most 128 boundary classes,
+// lx >= 3, the state buffer (9 lx pitch values) addressable with 32-bit byte
+// offsets, and its LDS (12 (cols + 1) storage values, 6 cols compute values,
+// the class table, one byte per row and wave column; cols = ly or a column
+// tile's width plus 2) within 160 KiB.
 bool sweep_supported(int storage, const Geom &g, const BcView &bc);
 // The batched sweep takes whole rows only (no column tiles).
 bool sweep_batch_supported(int storage, const Geom &g, const BcView &bc);
@@ -140,7 +141,6 @@ struct SweepJob {
     int64_t probe_cap = 0;
     int64_t probe_offset = 0;  // step-1 ring slot = batch step counter + probe_offset
     uint32_t rows = 0;         // output rows per block
-    uint32_t nchunks = 0;
     uint32_t block0 = 0;       // first block of the job in its bucket's grid
     uint32_t pad = 0;
 };
