@@ -114,5 +114,11 @@ def test_pending_checkpoint_and_rejections():
         b.restore(bytes(bad))
     with pytest.raises(LbmError):
         b.restore(blob[:100])
+    # format version 1 (rows not padded to 128-byte lines) is refused
+    assert int.from_bytes(blob[8:12], "little") == 2
+    old = bytearray(blob)
+    old[8:12] = (1).to_bytes(4, "little")
+    with pytest.raises(LbmError):
+        b.restore(bytes(old))
     for lat in (a, b, wrong, other):
         lat.close()
