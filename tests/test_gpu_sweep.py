@@ -178,3 +178,33 @@ def test_autotuned_launch_shape_gives_the_same_run():
         assert np.array_equal(x, y)
     a.close()
     b.close()
+
+
+def _random_cases(n=16, seed=7):
+    """Seeded shapes around the kernel's edges: rows just below or above a
+    multiple of the wavefront, of one block (1024 / 2048 columns) and of the
+    column-tile sizes."""
+    rng = np.random.default_rng(seed)
+    edges = [1, 2, 63, 65, 127, 129, 1023, 1025, 1030, 2047, 2049, 2111, 3070, 4093]
+    out = []
+    for _ in range(n):
+        ly = int(rng.choice(edges)) + int(rng.integers(-1, 2))
+        ly = max(ly, 1)
+        lx = int(rng.integers(3, 24))
+        storage = str(rng.choice(["f64", "f32", "f16"]))
+        kind = str(rng.choice(["periodic", "channel", "couette", "channel_open"]))
+        if kind != "periodic" and ly < 8:
+            kind = "periodic"
+        rows = int(rng.choice([0, 1, 3, 8]))
+        out.append((lx, ly, storage, kind, int(rng.integers(4, 9)), rows))
+    return out
+
+
+@pytest.mark.parametrize("lx,ly,storage,kind,steps,rows", _random_cases())
+def test_two_steps_equal_single_steps_random_shapes(lx, ly, storage, kind, steps, rows):
+    probe = (lx // 2, ly // 2)
+    (a, pa) = _run(lx, ly, storage, kind, steps, False, probe=probe, seed=lx * 7919 + ly)
+    (b, pb) = _run(lx, ly, storage, kind, steps, True, rows=rows, probe=probe, seed=lx * 7919 + ly)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert np.array_equal(pa, pb)
