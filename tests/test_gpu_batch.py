@@ -164,3 +164,41 @@ def test_batch_switching_between_one_and_two_steps_per_launch_keeps_probes():
         for a, c in zip(lat.state(), r.state()):
             assert np.array_equal(a, c)
     b.close()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_batched_sweep_edge_widths_equal_individual_runs(storage):
+    """Channels whose rows end just past a wavefront or a block (a whole wave of
+    lanes past the last column at ly = 1030) and, in FP64, rows wider than a
+    block, which the batch advances one step per launch beside the sweep
+    groups; every lattice equals its own single-step run."""
+    from lattice_boltzmann_parallel_solver_amd import Batch, Lattice
+    from lattice_boltzmann_parallel_solver_amd.boundary_utils import von_karman_boundary_conditions
+    from lattice_boltzmann_parallel_solver_amd.initial_values import density_1_velocity_x_u0_velocity_y_0_initial
+
+    shapes = [(9, 65), (7, 1030), (6, 1025), (5, 129), (8, 1100)]
+
+    def build():
+        lats, oms = [], []
+        for lx, ly in shapes:
+            d = max(2, ly // 4)
+            lat = Lattice(lx, ly, boundary=von_karman_boundary_conditions(lx, ly, 1.0, 0.1, d, walls=True, y_offset=1),
+                          storage=storage)
+            rho, u = density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), 0.1)
+            lat.set_equilibrium(rho, u)
+            lats.append(lat)
+            oms.append(1.0 / (3 * 0.1 * d / 100.0 + 0.5))
+        return lats, oms
+
+    lats, oms = build()
+    ref, _ = build()
+    b = Batch(lats, oms, temporal=2, rows_per_block=3)
+    for chunk in (9, 4):
+        b.run(chunk)
+        for lat, r, om in zip(lats, ref, oms):
+            r.run(om, chunk)
+            for a, c in zip(lat.state(), r.state()):
+                assert np.array_equal(a, c), (lat.lx, lat.ly)
+    b.close()
+    for lat in lats + ref:
+        lat.close()
