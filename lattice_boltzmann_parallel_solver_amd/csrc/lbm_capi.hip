@@ -720,6 +720,12 @@ struct lbm_batch {
     int sweep_rows = 8;
     void *d_sweep = nullptr;  // sweep job tables and block tables of the last run
     size_t sweep_cap = 0;
+    // side streams (default priority) that run the groups of a large batch
+    // side by side, with the events that fork them from and join them to
+    // `stream`
+    std::vector<hipStream_t> lanes;
+    std::vector<hipEvent_t> lane_done;
+    hipEvent_t fork = nullptr;
 };
 
 namespace {
@@ -742,13 +748,46 @@ void release_batch(lbm_batch *b) {
     (void)hipFree(b->d_step);
     (void)hipFree(b->d_done);
     (void)hipFree(b->d_sweep);
+    for (hipStream_t l : b->lanes) {
+        (void)hipStreamSynchronize(l);
+        (void)hipStreamDestroy(l);
+    }
+    for (hipEvent_t e : b->lane_done) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+// Side streams of a batch, and the batch size (nodes) from which its groups
+// run on them: below it a pair of launches is short and the graph replay
+// on one stream costs less than the extra launches.
+constexpr size_t kBatchLanes = 4;
+constexpr int64_t kBatchLaneNodes = 8 << 20;
+
+// At least `k` side streams (and their join events) for the batch.
+hipError_t ensure_lanes(lbm_batch *b, size_t k) {
+    if (!b->fork) {
+        hipError_t e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    while (b->lanes.size() < k) {
+        hipStream_t l = nullptr;
+        hipError_t e = hipStreamCreateWithFlags(&l, hipStreamNonBlocking);
+        if (e != hipSuccess) return e;
+        b->lanes.push_back(l);
+        hipEvent_t ev = nullptr;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+        b->lane_done.push_back(ev);
+    }
+    return hipSuccess;
 }
 
 // Steps of a batch with the two-step sweep kernel: jobs it supports are
 // grouped by block width (one launch per group per step pair), the others
-// take two one-step batched launches per pair.  Graph-captured pairs of pairs
-// replay the whole run; an odd last step goes job by job.
+// take two one-step batched launches per pair.  A large batch of several
+// groups runs each group on a side stream; otherwise graph-captured pairs of
+// pairs replay the whole run on the batch stream.  An odd last step goes job
+// by job.
 int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     const int n = (int)b->ctxs.size();
     struct Bucket {
@@ -884,7 +923,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             if (e != hipSuccess) break;
             e = launch_sweep2_batch(b->storage, bk.v, reinterpret_cast<const SweepJob *>(dev + bk.off_jobs[pp]),
                                     reinterpret_cast<const uint32_t *>(dev + bk.off_blocks),
-                                    (unsigned)bk.block_job.size(), bk.threads, bk.lds, b->d_step + 1, b->stream);
+                                    (unsigned)bk.block_job.size(), bk.threads, bk.lds, b->d_step + 1, 0, b->stream);
         }
         if (e == hipSuccess && !fjobs.empty())
             for (int p = 0; p < 2 && e == hipSuccess; p++)
@@ -897,7 +936,67 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     };
     const int64_t pairs = nsteps / 2;
     int64_t done_pairs = 0;
-    if (b->use_graph && pairs >= 2) {
+    // A large batch of several groups: each group (and the one-step jobs) on
+    // a side stream of its own, so that the tail of one group's launch
+    // overlaps the next group instead of idling the GPU (one stream runs its
+    // launches strictly one after another).  Groups touch disjoint lattices;
+    // the groups' probe slots come from the pair index by value, and the
+    // one-step jobs keep their counter on their own stream.
+    int64_t nodes = 0;
+    for (lbm_ctx *c : b->ctxs) nodes += c->g.N;
+    const size_t groups = buckets.size() + (fjobs.empty() ? 0 : 1);
+    if (groups > 1 && nodes >= (int64_t)kBatchLaneNodes && pairs >= 1) {
+        const size_t nl = std::min(groups, (size_t)kBatchLanes);
+        LBM_HIP_CHECK(ensure_lanes(b, nl), "lbm_batch_run: side streams");
+        // groups onto lanes, largest first onto the least loaded
+        std::vector<std::pair<int64_t, size_t>> load;  // (nodes, group)
+        for (size_t i = 0; i < buckets.size(); i++) {
+            int64_t g = 0;
+            for (int j : buckets[i].members) g += b->ctxs[j]->g.N;
+            load.push_back({g, i});
+        }
+        if (!fjobs.empty()) {
+            int64_t g = 0;
+            for (int j : fjobs) g += b->ctxs[j]->g.N;
+            load.push_back({g, buckets.size()});
+        }
+        std::sort(load.begin(), load.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+        std::vector<int64_t> lane_load(nl, 0);
+        std::vector<size_t> lane_of(groups, 0);
+        for (const auto &gl : load) {
+            const size_t l = (size_t)(std::min_element(lane_load.begin(), lane_load.end()) - lane_load.begin());
+            lane_of[gl.second] = l;
+            lane_load[l] += gl.first;
+        }
+        LBM_HIP_CHECK(hipEventRecord(b->fork, b->stream), "lbm_batch_run: fork");
+        for (size_t l = 0; l < nl; l++) LBM_HIP_CHECK(hipStreamWaitEvent(b->lanes[l], b->fork, 0), "lbm_batch_run: fork");
+        for (int64_t k = 0; k < pairs; k++) {
+            const int pp = (int)(k & 1);
+            for (size_t i = 0; i < buckets.size(); i++) {
+                const Bucket &bk = buckets[i];
+                LBM_HIP_CHECK(launch_sweep2_batch(b->storage, bk.v, reinterpret_cast<const SweepJob *>(dev + bk.off_jobs[pp]),
+                                                  reinterpret_cast<const uint32_t *>(dev + bk.off_blocks),
+                                                  (unsigned)bk.block_job.size(), bk.threads, bk.lds, nullptr,
+                                                  b->step_host + 2 * k, b->lanes[lane_of[i]]),
+                              "batched sweep pair");
+            }
+            if (!fjobs.empty())
+                for (int p = 0; p < 2; p++)
+                    LBM_HIP_CHECK(launch_fused_batch(b->storage, b->v, reinterpret_cast<const BatchJob *>(dev + off_f[p]),
+                                                     reinterpret_cast<const uint32_t *>(dev + off_fb),
+                                                     (unsigned)fblocks.size(), b->d_step, b->d_done,
+                                                     b->lanes[lane_of[buckets.size()]]),
+                                  "batched one-step pair");
+        }
+        for (size_t l = 0; l < nl; l++) {
+            LBM_HIP_CHECK(hipEventRecord(b->lane_done[l], b->lanes[l]), "lbm_batch_run: join");
+            LBM_HIP_CHECK(hipStreamWaitEvent(b->stream, b->lane_done[l], 0), "lbm_batch_run: join");
+        }
+        LBM_HIP_CHECK(launch_add_step(b->d_step + 1, 2 * pairs, b->stream), "lbm_batch_run: step counter");
+        if (fjobs.empty()) LBM_HIP_CHECK(launch_add_step(b->d_step, 2 * pairs, b->stream), "lbm_batch_run: step counter");
+        done_pairs = pairs;
+    }
+    if (b->use_graph && pairs - done_pairs >= 2) {
         LBM_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "graph capture");
         hipError_t e0 = pair(0);
         hipError_t e1 = e0 == hipSuccess ? pair(1) : e0;

@@ -1147,7 +1147,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
 template <class S, int V>
 __global__ __launch_bounds__(kSweepThreads) void k_sweep2_batch(const SweepJob *__restrict__ jobs,
                                                                 const uint32_t *__restrict__ block_job,
-                                                                const int64_t *__restrict__ step) {
+                                                                const int64_t *__restrict__ step, int64_t step_now) {
     using C = typename Io<S>::C;
     // consecutive blocks (neighbouring row ranges of a job) on one XCD, so the
     // halo rows they both read can hit that XCD's L2
@@ -1164,7 +1164,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2_batch(const SweepJob *
     p.probe_cell = j.probe_cell;
     p.probe1 = p.probe2 = nullptr;
     if (j.probe_cell >= 0) {
-        const int64_t slot = *step + j.probe_offset;
+        const int64_t slot = (step ? *step : step_now) + j.probe_offset;
         if (slot >= 0) p.probe1 = j.probe_ring + 3 * (slot % j.probe_cap);
         if (slot + 1 >= 0) p.probe2 = j.probe_ring + 3 * ((slot + 1) % j.probe_cap);
     }
@@ -1705,8 +1705,8 @@ hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int sr
 namespace {
 template <class S, int V>
 hipError_t sweep_batch_go(const SweepJob *jobs, const uint32_t *block_job, unsigned nblocks, unsigned threads,
-                          size_t lds, const int64_t *step, hipStream_t s) {
-    hipLaunchKernelGGL((k_sweep2_batch<S, V>), dim3(nblocks), dim3(threads), lds, s, jobs, block_job, step);
+                          size_t lds, const int64_t *step, int64_t step_now, hipStream_t s) {
+    hipLaunchKernelGGL((k_sweep2_batch<S, V>), dim3(nblocks), dim3(threads), lds, s, jobs, block_job, step, step_now);
     return hipGetLastError();
 }
 
@@ -1728,20 +1728,20 @@ hipError_t prepare_sweep2_batch(int storage, int v, size_t lds) {
 }
 
 hipError_t launch_sweep2_batch(int storage, int v, const SweepJob *jobs, const uint32_t *block_job, unsigned nblocks,
-                               unsigned threads, size_t lds, const int64_t *step, hipStream_t s) {
+                               unsigned threads, size_t lds, const int64_t *step, int64_t step_now, hipStream_t s) {
     if (nblocks == 0) return hipSuccess;
     if (threads == 0 || threads > (unsigned)kSweepThreads || threads % 64 != 0 || lds > kLdsLimit)
         return hipErrorInvalidValue;
     switch (storage) {
         case LBM_STORE_F64:
             if (v != 1) return hipErrorInvalidValue;
-            return sweep_batch_go<double, 1>(jobs, block_job, nblocks, threads, lds, step, s);
+            return sweep_batch_go<double, 1>(jobs, block_job, nblocks, threads, lds, step, step_now, s);
         case LBM_STORE_F32:
-            return v == 1 ? sweep_batch_go<float, 1>(jobs, block_job, nblocks, threads, lds, step, s)
-                          : sweep_batch_go<float, 2>(jobs, block_job, nblocks, threads, lds, step, s);
+            return v == 1 ? sweep_batch_go<float, 1>(jobs, block_job, nblocks, threads, lds, step, step_now, s)
+                          : sweep_batch_go<float, 2>(jobs, block_job, nblocks, threads, lds, step, step_now, s);
         case LBM_STORE_F16:
-            return v == 1 ? sweep_batch_go<__half, 1>(jobs, block_job, nblocks, threads, lds, step, s)
-                          : sweep_batch_go<__half, 2>(jobs, block_job, nblocks, threads, lds, step, s);
+            return v == 1 ? sweep_batch_go<__half, 1>(jobs, block_job, nblocks, threads, lds, step, step_now, s)
+                          : sweep_batch_go<__half, 2>(jobs, block_job, nblocks, threads, lds, step, step_now, s);
     }
     return hipErrorInvalidValue;
 }

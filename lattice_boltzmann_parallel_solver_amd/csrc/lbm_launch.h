@@ -147,8 +147,9 @@ struct SweepJob {
 
 // Two steps of every job of a bucket (all jobs with `threads` threads per
 // block and v columns per thread; lds = the largest layout among them).
+// Probe slots count from *step, or from step_now where step is null.
 hipError_t launch_sweep2_batch(int storage, int v, const SweepJob *jobs, const uint32_t *block_job, unsigned nblocks,
-                               unsigned threads, size_t lds, const int64_t *step, hipStream_t s);
+                               unsigned threads, size_t lds, const int64_t *step, int64_t step_now, hipStream_t s);
 // Allow `lds` bytes of dynamic LDS for the batched sweep kernel (call before
 // launching or capturing it).
 hipError_t prepare_sweep2_batch(int storage, int v, size_t lds);

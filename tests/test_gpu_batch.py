@@ -202,3 +202,49 @@ def test_batched_sweep_edge_widths_equal_individual_runs(storage):
     b.close()
     for lat in lats + ref:
         lat.close()
+
+
+def test_large_batch_groups_on_side_streams_equal_individual_runs():
+    """A batch above 8 Mi nodes with several groups and a pressure-fix job runs
+    its groups on side streams; states and probe traces still equal each
+    lattice's own single-step run, over several runs of odd and even length."""
+    from lattice_boltzmann_parallel_solver_amd import Batch, Lattice
+    from lattice_boltzmann_parallel_solver_amd.boundary_utils import (poiseuille_flow_boundary_conditions,
+                                                                      von_karman_boundary_conditions)
+    from lattice_boltzmann_parallel_solver_amd.initial_values import density_1_velocity_x_u0_velocity_y_0_initial
+
+    shapes = [(2048, 2048), (3000, 1000), (1600, 900), (700, 300), (900, 300)]
+
+    def build():
+        lats, oms = [], []
+        for lx, ly in shapes:
+            d = max(2, ly // 5)
+            lat = Lattice(lx, ly, boundary=von_karman_boundary_conditions(lx, ly, 1.0, 0.1, d, walls=True, y_offset=1),
+                          storage="f32")
+            rho, u = density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), 0.1)
+            lat.set_equilibrium(rho, u)
+            lat.set_probe(3 * lx // 4, ly // 2, 64)
+            lats.append(lat)
+            oms.append(1.0 / (3 * 0.1 * d / 100.0 + 0.5))
+        lat = Lattice(64, 32, boundary=poiseuille_flow_boundary_conditions(64, 32, 1.0015 / 3, 0.9985 / 3), storage="f32")
+        lat.set_equilibrium(np.ones((64, 32)), np.zeros((64, 32, 2)))
+        lat.set_probe(32, 16, 64)
+        lats.append(lat)
+        oms.append(1.5)
+        return lats, oms
+
+    lats, oms = build()
+    assert sum(l.lx * l.ly for l in lats) >= 8 << 20
+    ref, _ = build()
+    b = Batch(lats, oms, temporal=2)
+    for chunk in (6, 7, 4):
+        b.run(chunk)
+        for r, om in zip(ref, oms):
+            r.run(om, chunk)
+    b.close()
+    for lat, r in zip(lats, ref):
+        for a, c in zip(lat.state(), r.state()):
+            assert np.array_equal(a, c), (lat.lx, lat.ly)
+        assert np.array_equal(lat.probe(), r.probe()), (lat.lx, lat.ly)
+    for lat in lats + ref:
+        lat.close()
