@@ -21,8 +21,10 @@ fp64, f16  : the same workload with FP64 populations / FP16 deviation storage,
              tiles); fp64_single_step: FP64 with one step per launch.
 latency    : p50/p99 time-to-result of the continuous-batching server on a seeded
              Poisson job stream (scaled serving mix, stated in the line).
-batching   : aggregate MLUPS of seeded uneven channels advanced by one launch per
-             step versus the same channels solved one at a time.
+batching   : aggregate MLUPS of 256 seeded uneven channels advanced together (one
+             two-step launch per group of equal row width per step pair, the
+             groups on side streams) versus the same channels solved one at a
+             time, each at its autotuned launch shape.
 cpu_baseline: the oracle (numpy FP64) on a few steps of the same workload: P processes in
              the reference's block decomposition (value), one process, and the C
              restatement with OpenMP threads.
