@@ -165,7 +165,8 @@ class Server:
     nodes gave p50 24.9 s / p99 105.9 s at 116.4 GLUPS, against fifo's
     41.3 s / 106.2 s at 117.6 GLUPS.
     Results do not depend on the policy (a job's steps are the same in any
-    batch)."""
+    batch).  Under srpt a long job can wait while shorter ones keep arriving;
+    a finite stream always drains."""
 
     def __init__(self, hbm_budget: int = 160 << 30, chunk_steps: int = 2000, max_active: int = 256,
                  device: int = 0, nodes_per_thread: int = 1, temporal: int = 2, progress=None,
