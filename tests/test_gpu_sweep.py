@@ -208,3 +208,31 @@ def test_two_steps_equal_single_steps_random_shapes(lx, ly, storage, kind, steps
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
     assert np.array_equal(pa, pb)
+
+
+def test_autotuner_cache_keys_and_fallback():
+    """Lattices of one size bucket with and without 16-byte groups are tuned
+    apart, and a cached two-step shape falls back to one step on a lattice of
+    the same bucket that the sweep kernel refuses (the pressure fix)."""
+    from lattice_boltzmann_parallel_solver_amd import Lattice
+    from lattice_boltzmann_parallel_solver_amd import workloads as W
+    from lattice_boltzmann_parallel_solver_amd.boundary_utils import poiseuille_flow_boundary_conditions
+
+    W._LATTICE_CACHE.clear()
+    a = W.build_lattice(W.ChannelJob(120, 52, 12, 80.0), "f32")    # ny = 52: 16-byte groups
+    b = W.build_lattice(W.ChannelJob(120, 51, 12, 80.0), "f32")    # ny = 51: none
+    W.autotune_lattice(a, W.ChannelJob(120, 52, 12, 80.0).omega, steps=4)
+    W.autotune_lattice(b, W.ChannelJob(120, 51, 12, 80.0).omega, steps=4)
+    keys = sorted(W._LATTICE_CACHE)
+    assert len(keys) == 2 and {k[2] for k in keys} == {True, False}
+    # force a two-step entry for the bucket of a pressure-fix box and check the fallback
+    p = Lattice(120, 52, boundary=poiseuille_flow_boundary_conditions(120, 52, 1.0015 / 3, 0.9985 / 3), storage="f32")
+    p.set_equilibrium(np.ones((120, 52)), np.zeros((120, 52, 2)))
+    key = ("f32", W.size_bucket(120 * 52), True)
+    W._LATTICE_CACHE[key] = (1, True, False, 2, 1)
+    best = W.autotune_lattice(p, 1.4, steps=4)
+    assert best[3] == 1
+    p.run(1.4, 5)
+    for lat in (a, b, p):
+        lat.close()
+    W._LATTICE_CACHE.clear()
