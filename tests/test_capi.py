@@ -41,3 +41,18 @@ def test_check_raises_with_message():
     h = ctypes.c_void_p()
     with pytest.raises(_lib.LbmError, match="lattice"):
         _lib.check(L.lbm_create(ctypes.byref(h), -1, 5, 0, 0))
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """With the HIP library absent the package raises at first use; there is
+    no CPU fallback for any entry point."""
+    import lattice_boltzmann_parallel_solver_amd._lib as lib
+
+    monkeypatch.setattr(lib, "_lib", None)
+    monkeypatch.setattr(lib, "LIB_PATH", str(tmp_path / "liblbm_hip.so"))
+    with pytest.raises(RuntimeError, match="missing"):
+        lib.load()
+    from lattice_boltzmann_parallel_solver_amd import Lattice
+
+    with pytest.raises(RuntimeError, match="missing"):
+        Lattice(8, 8)
