@@ -59,12 +59,23 @@ class JobSpec:
         return self.omega
 
     def footprint(self) -> int:
-        """Device bytes of the job's context: two state buffers (rows padded to
-        128-byte lines), FP64 staging, boundary codes, probe ring."""
+        """Device bytes of the job's context, an upper bound: two state buffers
+        (rows padded to 128-byte lines), outlet buffers, the sweep's scratch
+        area, FP64 staging, reduction partials, boundary tables, probe ring,
+        and 2 MiB of rounding for each of its twelve allocations."""
         s = STORAGE_BYTES[self.storage]
-        pitch = -(-self.ly * s // 128) * 128 // s
+        lx, ly, n = self.lx, self.ly, self.nodes
+        pitch = -(-ly * s // 128) * 128 // s
+        waves = lx * -(-ly // 64)
+        planes = 2 * 9 * lx * pitch * s
+        sides = 2 * 3 * ly * max(s, 4)
+        scratch = 18 * 1024 * 8
+        staging = 96 * n
+        partials = 24 * max(-(-waves // 4), -(-n // 256)) + 24
+        special = 2 * lx + 2 * ly + 2 * self.plate + 16
+        tables = 2 * n + n // 64 + waves + 32 + 80 * 64 + 4 * special + 5 * 256
         probe = 24 * self.steps if self.kind == "von_karman" else 0
-        return 18 * s * self.lx * pitch + self.nodes * (96 + 2) + probe + (1 << 20)
+        return planes + sides + scratch + staging + partials + tables + probe + 12 * (2 << 20)
 
     def boundary(self):
         if self.kind == "von_karman":
