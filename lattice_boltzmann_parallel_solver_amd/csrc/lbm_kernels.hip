@@ -399,9 +399,9 @@ struct FusedParams {
 
 // Waves of 64 threads tile each row from y = 0 (the last wave of a row is
 // partly idle), so that with the line-aligned pitch every plane access of a
-// wave starts on a 64-element boundary: a row length that is not a multiple
-// of 32 nodes otherwise shifts most waves off the 128-byte lines (measured
-// 5-7 % slower at ly = 1800).  Wave w of the launch is row w / wpr, columns
+// wave starts on a 64-element boundary, and the lanes past the row end write
+// its padding: no 128-byte line is left partly written (rows of 1800 nodes
+// ran 5-7 % slower before).  Wave w of the launch is row w / wpr, columns
 // 64 (w mod wpr) ... + 63.
 template <class S, int V, bool kProbe, bool kReduce>
 __global__ __launch_bounds__(kBlock) void k_fused(FusedParams<S> p) {
