@@ -1,0 +1,72 @@
+"""Lattices whose state buffers exceed 4 GiB.
+
+The one-step kernel addresses populations with 64-bit offsets; the two-step
+kernel uses 32-bit byte offsets and declines such lattices.  A Couette channel
+(walls in y, periodic in x) whose initial state repeats a 64-row pattern keeps
+that period at every step, so a 64-row lattice run with the same inputs gives
+every row of the large one, bit for bit.  The 64-row run itself is checked
+against the oracle in test_gpu_lattice.py and test_gpu_sweep.py.
+"""
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PERIOD = 64
+LY = 2048
+
+
+def _couette(lx, ly):
+    from lattice_boltzmann_parallel_solver_amd import boundary_utils as BU
+
+    return BU.couette_flow_boundary_conditions(lx, ly, 0.05, 1.0)
+
+
+@pytest.mark.parametrize("storage,lx", [("f32", 65536), ("f64", 32768)])
+def test_state_above_4gib_equals_its_periodic_tile(storage, lx):
+    import torch
+
+    from lattice_boltzmann_parallel_solver_amd import Lattice
+    from lattice_boltzmann_parallel_solver_amd._lib import LbmError
+
+    elem = 4 if storage == "f32" else 8
+    assert 9 * lx * LY * elem >= 2**32  # one state buffer (pitch = ly here)
+    print(f"case: {storage} {lx}x{LY} (periodic tile {PERIOD}x{LY})", flush=True)
+    dev = torch.device("cuda", 0)
+    g = torch.Generator().manual_seed(7)
+    rho = 1.0 + 0.01 * torch.randn((PERIOD, LY), generator=g, dtype=torch.float64)
+    u = 0.05 * torch.randn((PERIOD, LY, 2), generator=g, dtype=torch.float64)
+    f = 1.0 / 9 + 1e-3 * torch.randn((PERIOD, LY, 9), generator=g, dtype=torch.float64)
+    rho, u, f = rho.to(dev), u.to(dev), f.to(dev)
+    steps, omega = 5, 1.3
+
+    small = Lattice(PERIOD, LY, boundary=_couette(PERIOD, LY), storage=storage)
+    small.set_state(f, rho, u)
+    small.run(omega, steps)
+    fs = torch.empty((PERIOD, LY, 9), dtype=torch.float64, device=dev)
+    rs = torch.empty((PERIOD, LY), dtype=torch.float64, device=dev)
+    us = torch.empty((PERIOD, LY, 2), dtype=torch.float64, device=dev)
+    small.state_into(fs, rs, us)
+    small.close()
+
+    reps = lx // PERIOD
+    big = Lattice(lx, LY, boundary=_couette(lx, LY), storage=storage)
+    with pytest.raises(LbmError, match="4 GiB"):
+        big.set_temporal(2)
+    big.set_state(f.repeat(reps, 1, 1), rho.repeat(reps, 1), u.repeat(reps, 1, 1))
+    big.run(omega, steps)
+    fb = torch.empty((lx, LY, 9), dtype=torch.float64, device=dev)
+    rb = torch.empty((lx, LY), dtype=torch.float64, device=dev)
+    ub = torch.empty((lx, LY, 2), dtype=torch.float64, device=dev)
+    big.state_into(fb, rb, ub)
+    big.close()
+    torch.cuda.synchronize()
+    # the tiles past the first 4 GiB of each state buffer are the ones the
+    # 64-bit offsets reach; every tile is compared
+    for name, b, s in (("f", fb, fs), ("rho", rb, rs), ("u", ub, us)):
+        t = b.view(reps, *s.shape)
+        bad = torch.nonzero((t != s.unsqueeze(0)).flatten(1).any(dim=1)).flatten()
+        assert bad.numel() == 0, f"{name}: tiles {bad[:8].tolist()} differ from the {PERIOD}-row lattice"
+    assert torch.isfinite(fs).all()
+    del fb, rb, ub
+    torch.cuda.empty_cache()
