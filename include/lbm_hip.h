@@ -165,9 +165,8 @@ int lbm_set_launch(lbm_ctx *ctx, int cells_per_thread, int xcd_remap, int nt_sto
  * each block of the two-step kernel sweeps (0 = automatic).  Rows wider than
  * one block (ny > 2048 FP32/FP16, > 1024 FP64) run as column tiles with a
  * halo column each side.  Two steps need no pressure fix in the boundary
- * program, nx >= 3, at most 128 boundary classes and 9*nx*pitch*sizeof
- * (storage) < 2^32 bytes; otherwise LBM_ERR_ARG.  No counterpart in the
- * reference (a launch setting of this library). */
+ * program, nx >= 3 and at most 128 boundary classes; otherwise LBM_ERR_ARG.
+ * No counterpart in the reference (a launch setting of this library). */
 int lbm_set_temporal(lbm_ctx *ctx, int steps_per_launch, int rows_per_block);
 
 /* ---- batches: many lattices advanced by one launch per step ------------- */

@@ -285,8 +285,8 @@ int lbm_set_temporal(lbm_ctx *c, int steps_per_launch, int rows_per_block) {
         return LBM_ERR_ARG;
     }
     if (steps_per_launch == 2 && !sweep_supported(c->storage, c->g, c->bc.view)) {
-        set_error("lbm_set_temporal: two steps per launch need no pressure fix, at most 128 boundary classes, "
-                  "nx >= 3 and state buffers below 4 GiB");
+        set_error("lbm_set_temporal: two steps per launch need no pressure fix, at most 128 boundary classes "
+                  "and nx >= 3");
         return LBM_ERR_ARG;
     }
     c->cfg.steps_per_launch = steps_per_launch;

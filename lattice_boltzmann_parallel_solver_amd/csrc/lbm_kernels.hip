@@ -145,8 +145,9 @@ __device__ __forceinline__ void pull(const S *__restrict__ src, const Geom &g, u
         out[i] = Io<S>::ld(src + pop(g, src_row(g, x, kCX[i]), i, src_col(g, y, kCY[i])));
 }
 
-// Element `off` of a state buffer through a 32-bit byte offset (lets the
-// compiler use the uniform-base + 32-bit-offset addressing form).  Valid while
+// Element `off` past `base` through a 32-bit byte offset (lets the compiler
+// use the uniform-base + 32-bit-offset addressing form).  Valid while
+// off * sizeof(S) < 2^32: for offsets from the start of a state buffer, while
 // 9 * lx * pitch * sizeof(S) < 2^32.
 template <class S>
 __device__ __forceinline__ const S *at32(const S *base, uint32_t off) {
@@ -157,8 +158,28 @@ __device__ __forceinline__ S *at32(S *base, uint32_t off) {
     return reinterpret_cast<S *>(reinterpret_cast<char *>(base) + (uint32_t)(off * (uint32_t)sizeof(S)));
 }
 
-// pull() with 32-bit offsets: the three source rows are given.
+// Plane i of row r of a state buffer, from a 64-bit base (wave-uniform where
+// r is); lanes add their column with at32().  For buffers of 4 GiB and more.
 template <class S>
+__device__ __forceinline__ const S *plane_row(const S *base, uint32_t pitch, uint32_t r, int i) {
+    return base + ((size_t)r * 9 + i) * pitch;
+}
+template <class S>
+__device__ __forceinline__ S *plane_row(S *base, uint32_t pitch, uint32_t r, int i) {
+    return base + ((size_t)r * 9 + i) * pitch;
+}
+
+// Population i of node (r, c): a 32-bit offset from the buffer start, or with
+// kBig a 64-bit row base plus a 32-bit column offset.  The 32-bit form keeps
+// fewer registers live in the row sweep (its offsets are not hoisted).
+template <bool kBig, class S>
+__device__ __forceinline__ S *pop32(S *base, uint32_t pitch, uint32_t r, int i, uint32_t c) {
+    if constexpr (kBig) return at32(plane_row(base, pitch, r, i), c);
+    else return at32(base, (r * 9 + i) * pitch + c);
+}
+
+// pull() with 32-bit offsets: the three source rows are given.
+template <class S, bool kBig = false>
 __device__ __forceinline__ void pull32(const S *__restrict__ src, uint32_t pitch, uint32_t xm, uint32_t x,
                                        uint32_t xp, uint32_t y, uint32_t ym, uint32_t yp,
                                        typename Io<S>::C out[9]) {
@@ -166,7 +187,7 @@ __device__ __forceinline__ void pull32(const S *__restrict__ src, uint32_t pitch
     for (int i = 0; i < 9; i++) {
         const uint32_t r = kCX[i] == 1 ? xm : (kCX[i] == -1 ? xp : x);
         const uint32_t c = kCY[i] == 1 ? ym : (kCY[i] == -1 ? yp : y);
-        out[i] = Io<S>::ld(at32(src, (r * 9 + i) * pitch + c));
+        out[i] = Io<S>::ld(pop32<kBig>(src, pitch, r, i, c));
     }
 }
 
@@ -782,7 +803,7 @@ __host__ __device__ inline SweepLds sweep_lds_layout(uint32_t cols, uint32_t ncl
 // pull32() for a node with a boundary class: directions the class reflects
 // load the node's own opposite population (the reflection's source) in place
 // of the upstream one, so that no load is needed later.
-template <class S>
+template <class S, bool kBig>
 __device__ __forceinline__ void pull32_bc(const S *__restrict__ src, uint32_t pitch, uint32_t xm, uint32_t x,
                                           uint32_t xp, uint32_t y, uint32_t ym, uint32_t yp, uint32_t w,
                                           typename Io<S>::C out[9]) {
@@ -791,8 +812,12 @@ __device__ __forceinline__ void pull32_bc(const S *__restrict__ src, uint32_t pi
         const uint32_t r = kCX[i] == 1 ? xm : (kCX[i] == -1 ? xp : x);
         const uint32_t c = kCY[i] == 1 ? ym : (kCY[i] == -1 ? yp : y);
         const bool refl = ((w >> (2 * i)) & 3u) == kSrcReflect;
-        const uint32_t off = refl ? (x * 9 + kOpp[i]) * pitch + y : (r * 9 + i) * pitch + c;
-        out[i] = Io<S>::ld(at32(src, off));
+        if constexpr (kBig) {
+            out[i] = Io<S>::ld(refl ? pop32<true>(src, pitch, x, kOpp[i], y) : pop32<true>(src, pitch, r, i, c));
+        } else {
+            const uint32_t off = refl ? (x * 9 + kOpp[i]) * pitch + y : (r * 9 + i) * pitch + c;
+            out[i] = Io<S>::ld(at32(src, off));
+        }
     }
 }
 
@@ -806,8 +831,9 @@ struct Flag {
 // tile * tw: step 1 also runs on one halo column each side (periodic in y),
 // whose values step 2 needs; the halo columns' step-2 results are not kept.
 // LDS and lane columns are then local: local column l is global column
-// cy0 - 1 + l (mod ly).
-template <class S, int V, bool kProbe, bool kTiled>
+// cy0 - 1 + l (mod ly).  kBig: state buffers of 4 GiB and more (64-bit row
+// bases).
+template <class S, int V, bool kProbe, bool kTiled, bool kBig>
 __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa, uint32_t tile) {
     using C = typename Io<S>::C;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -918,9 +944,9 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
         for (int k = 0; k < V; k++) {
             if (special(rr, k)) {
                 const uint32_t w = code[k] ? cls[code[k]].src : 0u;
-                pull32_bc<S>(p.src, pitch, xm, xr, xp, ys[k], ym[k], yp[k], w, out[k]);
+                pull32_bc<S, kBig>(p.src, pitch, xm, xr, xp, ys[k], ym[k], yp[k], w, out[k]);
             } else {
-                pull32<S>(p.src, pitch, xm, xr, xp, ys[k], ym[k], yp[k], out[k]);
+                pull32<S, kBig>(p.src, pitch, xm, xr, xp, ys[k], ym[k], yp[k], out[k]);
             }
         }
     };
@@ -1085,7 +1111,7 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
                     const uint32_t yst = min(threadIdx.x + k * T, pitch - 1);
 #pragma unroll
                     for (int q = 0; q < 9; q++)
-                        Io<S>::st(at32(p.dst, (r * 9 + q) * pitch + yst), o2.s[q], p.nt != 0);
+                        Io<S>::st(pop32<kBig>(p.dst, pitch, r, q, yst), o2.s[q], p.nt != 0);
                 }
                 if ((flags2 & kFlagOutletSide) && owned(k)) {
                     p.side_next[(size_t)y * 3 + 0] = v2[3];
@@ -1130,15 +1156,15 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
     body(R + 1, Flag<false>{}, Flag<true>{});
 }
 
-template <class S, int V, bool kProbe, bool kTiled>
+template <class S, int V, bool kProbe, bool kTiled, bool kBig>
 __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
     const uint32_t blk = p.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     if constexpr (kTiled) {
         // the tiles of one row range on consecutive blocks (one XCD's L2)
         const uint32_t rb = blk / p.ntiles;
-        sweep_block<S, V, kProbe, true>(p, rb * p.rows, blk - rb * p.ntiles);
+        sweep_block<S, V, kProbe, true, kBig>(p, rb * p.rows, blk - rb * p.ntiles);
     } else {
-        sweep_block<S, V, kProbe, false>(p, blk * p.rows, 0);
+        sweep_block<S, V, kProbe, false, kBig>(p, blk * p.rows, 0);
     }
 }
 
@@ -1174,7 +1200,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2_batch(const SweepJob *
     p.scratch = static_cast<S *>(j.scratch);
     p.ntiles = 1;
     p.tw = p.lw = j.g.ly;
-    sweep_block<S, V, true, false>(p, (blk - j.block0) * j.rows, 0);
+    sweep_block<S, V, true, false, false>(p, (blk - j.block0) * j.rows, 0);
 }
 
 __global__ void k_add_step(int64_t *step, int64_t k) { *step += k; }
@@ -1555,14 +1581,18 @@ size_t sweep_lds_for(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows)
     }
 }
 
-template <class S, int V, bool kProbe, bool kTiled>
-hipError_t sweep_go(const SweepParams<S> &p, unsigned grid, unsigned threads, size_t lds, hipStream_t s) {
-    auto fn = k_sweep2<S, V, kProbe, kTiled>;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(threads), lds, s, p);
-    return hipGetLastError();
+// The k_sweep2 instance of a launch.
+template <class S, int VV, bool kProbe>
+const void *sweep_fn_p(bool tiled, bool big) {
+    if (tiled)
+        return big ? reinterpret_cast<const void *>(k_sweep2<S, VV, kProbe, true, true>)
+                   : reinterpret_cast<const void *>(k_sweep2<S, VV, kProbe, true, false>);
+    return big ? reinterpret_cast<const void *>(k_sweep2<S, VV, kProbe, false, true>)
+               : reinterpret_cast<const void *>(k_sweep2<S, VV, kProbe, false, false>);
+}
+template <class S, int VV>
+const void *sweep_fn(bool probing, bool tiled, bool big) {
+    return probing ? sweep_fn_p<S, VV, true>(tiled, big) : sweep_fn_p<S, VV, false>(tiled, big);
 }
 
 template <class S>
@@ -1593,17 +1623,18 @@ hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView
     p.tw = sh.tw;
     p.lw = sh.lw;
     if (!p.scratch || (size_t)9 * V * threads * sizeof(S) > kSweepScratchBytes) return hipErrorInvalidValue;
+    // state buffers of 4 GiB and more take 64-bit row bases
+    const bool big = 9 * (uint64_t)g.lx * g.pitch * sizeof(S) >= (1ull << 32);
+    constexpr int V2 = sizeof(S) == 8 ? 1 : 2;  // FP64 rows are one column per thread
+    if (V != 1 && V != V2) return hipErrorInvalidValue;
+    auto pick = [&](bool probe) { return V == 1 ? sweep_fn<S, 1>(probe, tiled, big) : sweep_fn<S, V2>(probe, tiled, big); };
     int slots = 0;
     if (cfg.sweep_rows <= 0) {
         // resident blocks per CU (LDS and registers) times the CU count, in
         // row ranges (the tiles of a range are separate blocks)
         int dev = 0, cus = 0, per_cu = 0;
         const size_t lds_est = sweep_lds_layout<S>(sh.lw, bc.chunk ? bc.nclasses : 0, 64).total;
-        constexpr int V2 = sizeof(S) == 8 ? 1 : 2;
-        const void *fn = V == 1 ? (tiled ? reinterpret_cast<const void *>(k_sweep2<S, 1, false, true>)
-                                         : reinterpret_cast<const void *>(k_sweep2<S, 1, false, false>))
-                                : (tiled ? reinterpret_cast<const void *>(k_sweep2<S, V2, false, true>)
-                                         : reinterpret_cast<const void *>(k_sweep2<S, V2, false, false>));
+        const void *fn = pick(false);
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
             hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_est) == hipSuccess &&
@@ -1613,22 +1644,12 @@ hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView
     p.rows = (uint32_t)sweep_rows_for(g, cfg, slots);
     const unsigned grid = (g.lx + p.rows - 1) / p.rows * sh.ntiles;
     const size_t lds = sweep_lds_layout<S>(sh.lw, bc.chunk ? bc.nclasses : 0, p.rows).total;
-#define LBM_SWEEP(VV)                                                                     \
-    do {                                                                                  \
-        if (tiled)                                                                        \
-            return probing ? sweep_go<S, VV, true, true>(p, grid, threads, lds, s)        \
-                           : sweep_go<S, VV, false, true>(p, grid, threads, lds, s);      \
-        return probing ? sweep_go<S, VV, true, false>(p, grid, threads, lds, s)           \
-                       : sweep_go<S, VV, false, false>(p, grid, threads, lds, s);         \
-    } while (0)
-    if constexpr (sizeof(S) == 8) {
-        if (V != 1) return hipErrorInvalidValue;
-        LBM_SWEEP(1);
-    } else {
-        if (V == 1) LBM_SWEEP(1);
-        LBM_SWEEP(2);
-    }
-#undef LBM_SWEEP
+    const void *fn = pick(probing);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    void *args[] = {&p};
+    e = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, s);
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
 }  // namespace
@@ -1660,15 +1681,14 @@ bool sweep_supported(int storage, const Geom &g, const BcView &bc) {
     const SweepShape sh = sweep_shape(storage, g.ly);
     // every tile holds at least one output column
     if (sh.ntiles > 1 && (uint64_t)sh.tw * (sh.ntiles - 1) >= g.ly) return false;
-    // 9 lx pitch sizeof(S) bytes addressed with 32-bit offsets
-    if ((uint64_t)g.lx * g.pitch * 9 * (storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2)) >=
-        (1ull << 32))
-        return false;
     return sweep_lds_for(storage, sh.lw, bc.chunk ? bc.nclasses : 0, kSweepMaxRows) <= kLdsLimit;
 }
 
 bool sweep_batch_supported(int storage, const Geom &g, const BcView &bc) {
-    return sweep_supported(storage, g, bc) && sweep_shape(storage, g.ly).ntiles == 1;
+    // whole rows, state buffers below 4 GiB (32-bit offsets)
+    const uint64_t elem = storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2);
+    return sweep_supported(storage, g, bc) && sweep_shape(storage, g.ly).ntiles == 1 &&
+           9 * (uint64_t)g.lx * g.pitch * elem < (1ull << 32);
 }
 
 int sweep_rows_for(const Geom &g, const LaunchCfg &cfg, int slots) {
