@@ -14,7 +14,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 PERIOD = 64
-LY = 2048
+ELEM = {"f64": 8, "f32": 4, "f16": 2}
 
 
 def _couette(lx, ly):
@@ -23,41 +23,52 @@ def _couette(lx, ly):
     return BU.couette_flow_boundary_conditions(lx, ly, 0.05, 1.0)
 
 
-@pytest.mark.parametrize("storage,lx", [("f32", 65536), ("f64", 32768)])
-def test_state_above_4gib_equals_its_periodic_tile(storage, lx):
+# (storage, lx, ly): whole rows in every storage, and FP32 rows of 4096 nodes
+# (three column tiles in the two-step kernel)
+CASES = [("f32", 65536, 2048), ("f64", 32768, 2048), ("f16", 131072, 2048), ("f32", 32768, 4096)]
+
+
+@pytest.mark.parametrize("storage,lx,ly", CASES)
+def test_state_above_4gib_equals_its_periodic_tile(storage, lx, ly):
     import torch
 
     from lattice_boltzmann_parallel_solver_amd import Lattice
 
-    elem = 4 if storage == "f32" else 8
-    assert 9 * lx * LY * elem >= 2**32  # one state buffer (pitch = ly here)
-    print(f"case: {storage} {lx}x{LY} (periodic tile {PERIOD}x{LY})", flush=True)
+    assert 9 * lx * ly * ELEM[storage] >= 2**32  # one state buffer (pitch = ly here)
+    print(f"case: {storage} {lx}x{ly} (periodic tile {PERIOD}x{ly})", flush=True)
     dev = torch.device("cuda", 0)
     g = torch.Generator().manual_seed(7)
-    rho = 1.0 + 0.01 * torch.randn((PERIOD, LY), generator=g, dtype=torch.float64)
-    u = 0.05 * torch.randn((PERIOD, LY, 2), generator=g, dtype=torch.float64)
-    f = 1.0 / 9 + 1e-3 * torch.randn((PERIOD, LY, 9), generator=g, dtype=torch.float64)
+    rho = 1.0 + 0.01 * torch.randn((PERIOD, ly), generator=g, dtype=torch.float64)
+    u = 0.05 * torch.randn((PERIOD, ly, 2), generator=g, dtype=torch.float64)
+    f = 1.0 / 9 + 1e-3 * torch.randn((PERIOD, ly, 9), generator=g, dtype=torch.float64)
     rho, u, f = rho.to(dev), u.to(dev), f.to(dev)
     steps, omega = 5, 1.3
+    # a probe in the last tile of the large lattice (past 4 GiB) and at the
+    # same place in the 64-row lattice
+    px, py = 5, ly // 2 + 3
 
-    small = Lattice(PERIOD, LY, boundary=_couette(PERIOD, LY), storage=storage)
+    small = Lattice(PERIOD, ly, boundary=_couette(PERIOD, ly), storage=storage)
     small.set_state(f, rho, u)
+    small.set_probe(px, py, steps)
     small.run(omega, steps)
-    fs = torch.empty((PERIOD, LY, 9), dtype=torch.float64, device=dev)
-    rs = torch.empty((PERIOD, LY), dtype=torch.float64, device=dev)
-    us = torch.empty((PERIOD, LY, 2), dtype=torch.float64, device=dev)
+    fs = torch.empty((PERIOD, ly, 9), dtype=torch.float64, device=dev)
+    rs = torch.empty((PERIOD, ly), dtype=torch.float64, device=dev)
+    us = torch.empty((PERIOD, ly, 2), dtype=torch.float64, device=dev)
     small.state_into(fs, rs, us)
+    ps = small.probe()
     small.close()
+    assert ps.shape == (steps, 3)
 
     reps = lx // PERIOD
-    big = Lattice(lx, LY, boundary=_couette(lx, LY), storage=storage)
-    fb = torch.empty((lx, LY, 9), dtype=torch.float64, device=dev)
-    rb = torch.empty((lx, LY), dtype=torch.float64, device=dev)
-    ub = torch.empty((lx, LY, 2), dtype=torch.float64, device=dev)
+    big = Lattice(lx, ly, boundary=_couette(lx, ly), storage=storage)
+    fb = torch.empty((lx, ly, 9), dtype=torch.float64, device=dev)
+    rb = torch.empty((lx, ly), dtype=torch.float64, device=dev)
+    ub = torch.empty((lx, ly, 2), dtype=torch.float64, device=dev)
     for temporal in (1, 2):
-        print(f"case: {storage} {lx}x{LY}, {temporal} step(s) per launch", flush=True)
+        print(f"case: {storage} {lx}x{ly}, {temporal} step(s) per launch", flush=True)
         big.set_temporal(temporal)
         big.set_state(f.repeat(reps, 1, 1), rho.repeat(reps, 1), u.repeat(reps, 1, 1))
+        big.set_probe(lx - PERIOD + px, py, steps)
         big.run(omega, steps)
         big.state_into(fb, rb, ub)
         torch.cuda.synchronize()
@@ -68,6 +79,8 @@ def test_state_above_4gib_equals_its_periodic_tile(storage, lx):
             bad = torch.nonzero((t != s.unsqueeze(0)).flatten(1).any(dim=1)).flatten()
             assert bad.numel() == 0, (f"{name} ({temporal} step(s) per launch): tiles {bad[:8].tolist()} differ "
                                       f"from the {PERIOD}-row lattice")
+        pb = big.probe()
+        assert pb.shape == ps.shape and (pb == ps).all(), f"probe ({temporal} step(s) per launch)"
     big.close()
     assert torch.isfinite(fs).all()
     del fb, rb, ub
