@@ -17,10 +17,7 @@ from typing import Sequence
 
 import numpy as np
 
-from . import _lib
-from ._device import stream, to_device, torch_cuda
-from ._lib import check
-from .boundary_conditions import (BoundaryRule, Inlet, Outlet, PeriodicPressure, inlet, moving_wall, outlet,
+from .boundary_conditions import (BoundaryRule, PeriodicPressure, inlet, moving_wall, outlet,
                                   periodic_with_pressure_variations, rigid_object, rigid_wall)
 
 

@@ -13,7 +13,7 @@ reference steps, ``state()`` returns (f_n, rho_n, u_n).
 from __future__ import annotations
 
 import ctypes
-from typing import Iterable, Optional, Sequence, Tuple
+from typing import Optional, Sequence, Tuple
 
 import numpy as np
 

@@ -55,7 +55,7 @@ How each part is pinned (tests/test_oracle_*.py)
 from __future__ import annotations
 
 import math
-from typing import Callable, Optional, Sequence, Tuple
+from typing import Callable, Optional, Tuple
 
 import numpy as np
 
