@@ -793,6 +793,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     struct Bucket {
         int v = 1;
         unsigned threads = 0;
+        bool big = false;  // state buffers of 4 GiB or more
         size_t lds = 0;
         std::vector<int> members;
         std::vector<uint32_t> block_job;  // index into members
@@ -809,14 +810,16 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
         const SweepShape sh = sweep_shape(c->storage, c->g.ly);  // whole rows here
         const int v = sh.v;
         const unsigned threads = sh.threads;
+        const bool big = sweep_big(c->storage, c->g);
         Bucket *bk = nullptr;
         for (Bucket &x : buckets)
-            if (x.v == v && x.threads == threads) bk = &x;
+            if (x.v == v && x.threads == threads && x.big == big) bk = &x;
         if (!bk) {
             buckets.emplace_back();
             bk = &buckets.back();
             bk->v = v;
             bk->threads = threads;
+            bk->big = big;
         }
         bk->members.push_back(j);
     }
@@ -901,7 +904,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
         off_fb = put(fblocks.data(), fblocks.size() * sizeof(uint32_t));
     }
     for (const Bucket &bk : buckets)
-        LBM_HIP_CHECK(prepare_sweep2_batch(b->storage, bk.v, bk.lds), "lbm_batch_run: sweep kernel LDS");
+        LBM_HIP_CHECK(prepare_sweep2_batch(b->storage, bk.v, bk.big, bk.lds), "lbm_batch_run: sweep kernel LDS");
     drop_graph(b);  // the one-step graph of an earlier run would hold stale tables
     if (host.size() > b->sweep_cap) {
         (void)hipFree(b->d_sweep);
@@ -921,7 +924,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
         hipError_t e = hipSuccess;
         for (const Bucket &bk : buckets) {
             if (e != hipSuccess) break;
-            e = launch_sweep2_batch(b->storage, bk.v, reinterpret_cast<const SweepJob *>(dev + bk.off_jobs[pp]),
+            e = launch_sweep2_batch(b->storage, bk.v, bk.big, reinterpret_cast<const SweepJob *>(dev + bk.off_jobs[pp]),
                                     reinterpret_cast<const uint32_t *>(dev + bk.off_blocks),
                                     (unsigned)bk.block_job.size(), bk.threads, bk.lds, b->d_step + 1, 0, b->stream);
         }
@@ -974,7 +977,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             const int pp = (int)(k & 1);
             for (size_t i = 0; i < buckets.size(); i++) {
                 const Bucket &bk = buckets[i];
-                LBM_HIP_CHECK(launch_sweep2_batch(b->storage, bk.v, reinterpret_cast<const SweepJob *>(dev + bk.off_jobs[pp]),
+                LBM_HIP_CHECK(launch_sweep2_batch(b->storage, bk.v, bk.big, reinterpret_cast<const SweepJob *>(dev + bk.off_jobs[pp]),
                                                   reinterpret_cast<const uint32_t *>(dev + bk.off_blocks),
                                                   (unsigned)bk.block_job.size(), bk.threads, bk.lds, nullptr,
                                                   b->step_host + 2 * k, b->lanes[lane_of[i]]),

@@ -1169,8 +1169,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
 }
 
 // Two steps of every job of a batch bucket: block b sweeps rows of job
-// block_job[b]; probe samples go to ring slots *step + offset (+1).
-template <class S, int V>
+// block_job[b]; probe samples go to ring slots *step + offset (+1).  kBig:
+// the bucket's state buffers are 4 GiB or more (64-bit row bases).
+template <class S, int V, bool kBig>
 __global__ __launch_bounds__(kSweepThreads) void k_sweep2_batch(const SweepJob *__restrict__ jobs,
                                                                 const uint32_t *__restrict__ block_job,
                                                                 const int64_t *__restrict__ step, int64_t step_now) {
@@ -1200,7 +1201,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2_batch(const SweepJob *
     p.scratch = static_cast<S *>(j.scratch);
     p.ntiles = 1;
     p.tw = p.lw = j.g.ly;
-    sweep_block<S, V, true, false, false>(p, (blk - j.block0) * j.rows, 0);
+    sweep_block<S, V, true, false, kBig>(p, (blk - j.block0) * j.rows, 0);
 }
 
 __global__ void k_add_step(int64_t *step, int64_t k) { *step += k; }
@@ -1624,7 +1625,7 @@ hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView
     p.lw = sh.lw;
     if (!p.scratch || (size_t)9 * V * threads * sizeof(S) > kSweepScratchBytes) return hipErrorInvalidValue;
     // state buffers of 4 GiB and more take 64-bit row bases
-    const bool big = 9 * (uint64_t)g.lx * g.pitch * sizeof(S) >= (1ull << 32);
+    const bool big = sweep_big(sizeof(S) == 8 ? LBM_STORE_F64 : (sizeof(S) == 4 ? LBM_STORE_F32 : LBM_STORE_F16), g);
     constexpr int V2 = sizeof(S) == 8 ? 1 : 2;  // FP64 rows are one column per thread
     if (V != 1 && V != V2) return hipErrorInvalidValue;
     auto pick = [&](bool probe) { return V == 1 ? sweep_fn<S, 1>(probe, tiled, big) : sweep_fn<S, V2>(probe, tiled, big); };
@@ -1685,10 +1686,8 @@ bool sweep_supported(int storage, const Geom &g, const BcView &bc) {
 }
 
 bool sweep_batch_supported(int storage, const Geom &g, const BcView &bc) {
-    // whole rows, state buffers below 4 GiB (32-bit offsets)
-    const uint64_t elem = storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2);
-    return sweep_supported(storage, g, bc) && sweep_shape(storage, g.ly).ntiles == 1 &&
-           9 * (uint64_t)g.lx * g.pitch * elem < (1ull << 32);
+    // whole rows only
+    return sweep_supported(storage, g, bc) && sweep_shape(storage, g.ly).ntiles == 1;
 }
 
 int sweep_rows_for(const Geom &g, const LaunchCfg &cfg, int slots) {
@@ -1723,47 +1722,47 @@ hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int sr
 }
 
 namespace {
-template <class S, int V>
-hipError_t sweep_batch_go(const SweepJob *jobs, const uint32_t *block_job, unsigned nblocks, unsigned threads,
-                          size_t lds, const int64_t *step, int64_t step_now, hipStream_t s) {
-    hipLaunchKernelGGL((k_sweep2_batch<S, V>), dim3(nblocks), dim3(threads), lds, s, jobs, block_job, step, step_now);
-    return hipGetLastError();
-}
-
-template <class S, int V>
-hipError_t sweep_batch_attr(size_t lds) {
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(k_sweep2_batch<S, V>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+// The k_sweep2_batch instance of a bucket, or null for a shape it does not have.
+const void *sweep_batch_fn(int storage, int v, bool big) {
+    auto pick = [&](auto small_fn, auto big_fn) { return big ? reinterpret_cast<const void *>(big_fn)
+                                                             : reinterpret_cast<const void *>(small_fn); };
+    switch (storage) {
+        case LBM_STORE_F64:
+            return v == 1 ? pick(k_sweep2_batch<double, 1, false>, k_sweep2_batch<double, 1, true>) : nullptr;
+        case LBM_STORE_F32:
+            return v == 1 ? pick(k_sweep2_batch<float, 1, false>, k_sweep2_batch<float, 1, true>)
+                          : (v == 2 ? pick(k_sweep2_batch<float, 2, false>, k_sweep2_batch<float, 2, true>) : nullptr);
+        case LBM_STORE_F16:
+            return v == 1 ? pick(k_sweep2_batch<__half, 1, false>, k_sweep2_batch<__half, 1, true>)
+                          : (v == 2 ? pick(k_sweep2_batch<__half, 2, false>, k_sweep2_batch<__half, 2, true>) : nullptr);
+    }
+    return nullptr;
 }
 }  // namespace
 
-hipError_t prepare_sweep2_batch(int storage, int v, size_t lds) {
-    if (lds > kLdsLimit) return hipErrorInvalidValue;
-    switch (storage) {
-        case LBM_STORE_F64: return v == 1 ? sweep_batch_attr<double, 1>(lds) : hipErrorInvalidValue;
-        case LBM_STORE_F32: return v == 1 ? sweep_batch_attr<float, 1>(lds) : sweep_batch_attr<float, 2>(lds);
-        case LBM_STORE_F16: return v == 1 ? sweep_batch_attr<__half, 1>(lds) : sweep_batch_attr<__half, 2>(lds);
-    }
-    return hipErrorInvalidValue;
+bool sweep_big(int storage, const Geom &g) {
+    const uint64_t elem = storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2);
+    return 9 * (uint64_t)g.lx * g.pitch * elem >= (1ull << 32);
 }
 
-hipError_t launch_sweep2_batch(int storage, int v, const SweepJob *jobs, const uint32_t *block_job, unsigned nblocks,
-                               unsigned threads, size_t lds, const int64_t *step, int64_t step_now, hipStream_t s) {
+hipError_t prepare_sweep2_batch(int storage, int v, bool big, size_t lds) {
+    if (lds > kLdsLimit) return hipErrorInvalidValue;
+    const void *fn = sweep_batch_fn(storage, v, big);
+    if (!fn) return hipErrorInvalidValue;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+hipError_t launch_sweep2_batch(int storage, int v, bool big, const SweepJob *jobs, const uint32_t *block_job,
+                               unsigned nblocks, unsigned threads, size_t lds, const int64_t *step, int64_t step_now,
+                               hipStream_t s) {
     if (nblocks == 0) return hipSuccess;
     if (threads == 0 || threads > (unsigned)kSweepThreads || threads % 64 != 0 || lds > kLdsLimit)
         return hipErrorInvalidValue;
-    switch (storage) {
-        case LBM_STORE_F64:
-            if (v != 1) return hipErrorInvalidValue;
-            return sweep_batch_go<double, 1>(jobs, block_job, nblocks, threads, lds, step, step_now, s);
-        case LBM_STORE_F32:
-            return v == 1 ? sweep_batch_go<float, 1>(jobs, block_job, nblocks, threads, lds, step, step_now, s)
-                          : sweep_batch_go<float, 2>(jobs, block_job, nblocks, threads, lds, step, step_now, s);
-        case LBM_STORE_F16:
-            return v == 1 ? sweep_batch_go<__half, 1>(jobs, block_job, nblocks, threads, lds, step, step_now, s)
-                          : sweep_batch_go<__half, 2>(jobs, block_job, nblocks, threads, lds, step, step_now, s);
-    }
-    return hipErrorInvalidValue;
+    const void *fn = sweep_batch_fn(storage, v, big);
+    if (!fn) return hipErrorInvalidValue;
+    void *args[] = {&jobs, &block_job, &step, &step_now};
+    hipError_t e = hipLaunchKernel(fn, dim3(nblocks), dim3(threads), args, lds, s);
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
 hipError_t launch_add_step(int64_t *step, int64_t k, hipStream_t s) {

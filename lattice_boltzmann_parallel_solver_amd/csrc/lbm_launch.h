@@ -148,11 +148,16 @@ struct SweepJob {
 // Two steps of every job of a bucket (all jobs with `threads` threads per
 // block and v columns per thread; lds = the largest layout among them).
 // Probe slots count from *step, or from step_now where step is null.
-hipError_t launch_sweep2_batch(int storage, int v, const SweepJob *jobs, const uint32_t *block_job, unsigned nblocks,
-                               unsigned threads, size_t lds, const int64_t *step, int64_t step_now, hipStream_t s);
+// big: the bucket's state buffers are 4 GiB or more (sweep_big).
+hipError_t launch_sweep2_batch(int storage, int v, bool big, const SweepJob *jobs, const uint32_t *block_job,
+                               unsigned nblocks, unsigned threads, size_t lds, const int64_t *step, int64_t step_now,
+                               hipStream_t s);
 // Allow `lds` bytes of dynamic LDS for the batched sweep kernel (call before
 // launching or capturing it).
-hipError_t prepare_sweep2_batch(int storage, int v, size_t lds);
+hipError_t prepare_sweep2_batch(int storage, int v, bool big, size_t lds);
+// Whether a lattice's state buffers need 64-bit row bases in the sweep
+// kernels (4 GiB or more each).
+bool sweep_big(int storage, const Geom &g);
 // LDS bytes of the sweep kernel for a lattice of this ly, class count and rows per block.
 size_t sweep_lds_bytes(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows);
 // *step += k (one thread).

@@ -85,3 +85,55 @@ def test_state_above_4gib_equals_its_periodic_tile(storage, lx, ly):
     assert torch.isfinite(fs).all()
     del fb, rb, ub
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("with_small", [False, True])
+def test_batch_with_a_job_above_4gib_equals_its_periodic_tile(with_small):
+    """A batch of a 65536x2048 FP32 channel (two steps per launch with 64-bit
+    row bases), alone (one group, graph-replayed pairs) or with a 64-row
+    channel of the same repeating inputs (two groups on side streams): every
+    tile of the large one equals the 64-row lattice run on its own."""
+    import torch
+
+    from lattice_boltzmann_parallel_solver_amd import Batch, Lattice
+
+    storage, lx, ly = "f32", 65536, 2048
+    print(f"case: batch of {storage} {lx}x{ly}" + (f" and {PERIOD}x{ly}" if with_small else ""), flush=True)
+    dev = torch.device("cuda", 0)
+    g = torch.Generator().manual_seed(11)
+    rho = (1.0 + 0.01 * torch.randn((PERIOD, ly), generator=g, dtype=torch.float64)).to(dev)
+    u = (0.05 * torch.randn((PERIOD, ly, 2), generator=g, dtype=torch.float64)).to(dev)
+    f = (1.0 / 9 + 1e-3 * torch.randn((PERIOD, ly, 9), generator=g, dtype=torch.float64)).to(dev)
+    steps, omega = 6, 1.3
+
+    ref = Lattice(PERIOD, ly, boundary=_couette(PERIOD, ly), storage=storage)
+    ref.set_state(f, rho, u)
+    ref.run(omega, steps)
+    fs = torch.empty((PERIOD, ly, 9), dtype=torch.float64, device=dev)
+    ref.state_into(fs)
+    ref.close()
+
+    reps = lx // PERIOD
+    big = Lattice(lx, ly, boundary=_couette(lx, ly), storage=storage)
+    big.set_state(f.repeat(reps, 1, 1), rho.repeat(reps, 1), u.repeat(reps, 1, 1))
+    lats = [big]
+    if with_small:
+        small = Lattice(PERIOD, ly, boundary=_couette(PERIOD, ly), storage=storage)
+        small.set_state(f, rho, u)
+        lats.append(small)
+    batch = Batch(lats, [omega] * len(lats), temporal=2)
+    batch.run(steps)
+    batch.close()
+    fb = torch.empty((lx, ly, 9), dtype=torch.float64, device=dev)
+    big.state_into(fb)
+    big.close()
+    if with_small:
+        f2 = torch.empty((PERIOD, ly, 9), dtype=torch.float64, device=dev)
+        small.state_into(f2)
+        small.close()
+        assert torch.equal(f2, fs)
+    torch.cuda.synchronize()
+    bad = torch.nonzero((fb.view(reps, PERIOD, ly, 9) != fs.unsqueeze(0)).flatten(1).any(dim=1)).flatten()
+    assert bad.numel() == 0, f"tiles {bad[:8].tolist()} differ from the {PERIOD}-row lattice"
+    del fb
+    torch.cuda.empty_cache()
