@@ -256,7 +256,7 @@ class Server:
             self.busy += time.perf_counter() - tb
             if self.progress is not None:
                 self.progress(time.perf_counter() - t0, len(active), len(waiting) + len(incoming),
-                              sum(1 for r in records if r.finished == r.finished))
+                              sum(1 for r in records if not math.isnan(r.finished)))
             finished = []
             for i in ids:
                 rec, lat, left = active[i]
