@@ -14,6 +14,8 @@ step and returns the numbers the reference plotted or wrote to CSV.
   strouhal_interpolated,
   strouhal_zero_crossing   sub-bin estimators (the reference's peak is quantised
                            to one FFT bin, d/(u N) in Strouhal units)
+  step_rates               node rate with one and with two steps per launch on a
+                           lattice of any size (no counterpart: a measurement)
 """
 
 from __future__ import annotations
@@ -349,6 +351,51 @@ def channel_checks(lx: int = 4200, ly: int = 1800, plate: int = 400, steps: int 
     return out
 
 
+def step_rates(lx: int, ly: int, boundary: str = "couette", storages=("f32", "f64"), steps: int = 200,
+               omega: float = 1.0, device: int = 0) -> dict:
+    """Node rate (GLUPS) of lbm_run with one and with two steps per launch.
+
+    The lattice starts at equilibrium with rho = 1, u = (0.05, 0); boundary is
+    "periodic", "couette" (lid 0.05) or "channel" (von Karman channel with
+    walls, plate ly/5).  Each rate is the best of two timed runs of `steps`
+    steps after 10 untimed ones; `state_buffer_bytes` is the size of one of
+    the two population buffers (rows padded to 128 bytes)."""
+    import time
+
+    bcs = {"periodic": lambda: None,
+           "couette": lambda: couette_flow_boundary_conditions(lx, ly, 0.05, 1.0),
+           "channel": lambda: von_karman_boundary_conditions(lx, ly, 1.0, 0.1, max(2, ly // 5), walls=True,
+                                                             y_offset=1)}
+    if boundary not in bcs:
+        raise ValueError(f"boundary must be one of {sorted(bcs)}")
+    rho = np.ones((lx, ly))
+    u = np.zeros((lx, ly, 2))
+    u[..., 0] = 0.05
+    out = {"lattice": [lx, ly], "boundary": boundary, "steps": steps, "omega": omega, "state_buffer_bytes": {},
+           "glups": {}}
+    for st in storages:
+        elem = {"f64": 8, "f32": 4, "f16": 2}[st]
+        pitch = -(-ly * elem // 128) * 128 // elem
+        lat = Lattice(lx, ly, boundary=bcs[boundary](), storage=st, device=device)
+        lat.set_equilibrium(rho, u)
+        rates = {}
+        for spl in (1, 2):
+            lat.set_temporal(spl)
+            best = 0.0
+            for _ in range(2):
+                lat.run(omega, 10)
+                lat.synchronize()
+                t0 = time.perf_counter()
+                lat.run(omega, steps)
+                lat.synchronize()
+                best = max(best, lx * ly * steps / (time.perf_counter() - t0) / 1e9)
+            rates[f"{spl}_per_launch"] = best
+        lat.close()
+        out["glups"][st] = rates
+        out["state_buffer_bytes"][st] = 9 * lx * pitch * elem
+    return out
+
+
 def _main(argv=None):
     import argparse
     import json
@@ -370,7 +417,21 @@ def _main(argv=None):
     c.add_argument("--box", type=int, default=2048)
     c.add_argument("--box-steps", type=int, default=50000)
     c.add_argument("--out", default=None)
+    r = sub.add_parser("rates", help="node rate with one and two steps per launch on a lattice of any size")
+    r.add_argument("--lx", type=int, default=8192)
+    r.add_argument("--ly", type=int, default=2048)
+    r.add_argument("--boundary", default="couette", choices=["periodic", "couette", "channel"])
+    r.add_argument("--storages", nargs="+", default=["f32", "f64"], choices=["f64", "f32", "f16"])
+    r.add_argument("--steps", type=int, default=200)
+    r.add_argument("--out", default=None)
     a = ap.parse_args(argv)
+    if a.cmd == "rates":
+        res = step_rates(a.lx, a.ly, a.boundary, a.storages, a.steps)
+        text = json.dumps(res, indent=1)
+        print(text)
+        if a.out:
+            with open(a.out, "w") as fh:
+                fh.write(text)
     if a.cmd == "channel-checks":
         res = channel_checks(a.lx, a.ly, a.plate, a.steps, a.box, a.box_steps,
                              progress=lambda tag, n: print(f"{tag} step {n}", flush=True))
