@@ -727,7 +727,8 @@ __global__ void k_next_step(int64_t *step) { *step += 1; }
 // the result equals two k_fused launches bit for bit.  The input is read once
 // (plus the two rows on each side that neighbouring blocks also read) and the
 // output written once per two steps.  The step-1 loads of the next row are
-// issued before the current row is processed.  Not used with the pressure fix,
+// issued before the current row is processed (in FP32, half of them after the
+// row's barrier).  Not used with the pressure fix,
 // whose producers write rows owned by other blocks.
 // ---------------------------------------------------------------------------
 constexpr int kSweepThreads = 1024;
@@ -983,7 +984,8 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
     pull_row(0, c0, pf, K0{}, KV{});
 
     // One iteration: step 1 on row x = row_at(it) (prefetching the next row
-    // first if kPf), then, if kS2, step 2 on row x - 1.
+    // first if kPf, its second column after the barrier where kH < V), then,
+    // if kS2, step 2 on row x - 1.
     auto body = [&](uint32_t it, auto pf_flag, auto s2_flag) {
         constexpr bool kPf = decltype(pf_flag)::value;
         constexpr bool kS2 = decltype(s2_flag)::value;
