@@ -173,8 +173,8 @@ class Server:
     keep the GPU near full rate.  Short jobs then finish without sharing the
     GPU with long ones: lower median time-to-result for the same total work.
     On the 40-job stream of 8192x2048 channels and squares, srpt with 96 Mi
-    nodes gave p50 24.9 s / p99 105.9 s at 116.4 GLUPS, against fifo's
-    41.3 s / 106.2 s at 117.6 GLUPS.
+    nodes gave p50 24.7 s / p99 104.4 s at 118.2 GLUPS, against fifo's
+    40.6 s / 104.5 s at 119.5 GLUPS.
     Results do not depend on the policy (a job's steps are the same in any
     batch).  Under srpt a long job can wait while shorter ones keep arriving;
     a finite stream always drains."""
