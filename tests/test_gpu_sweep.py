@@ -78,6 +78,9 @@ CASES = [
     (26, 200, "f32", "couette", 10, 6),
     (26, 200, "f16", "periodic", 10, 100),
     (16, 1030, "f32", "channel", 9, 4),        # a whole wave of lanes past the last column (walls there)
+    (6, 2048, "f32", "couette", 7, 1),         # two columns per thread (second pulled after the barrier), 1 row
+    (10, 1500, "f32", "channel", 8, 2),        # the same with 2 rows per block
+    (9, 1800, "f32", "channel", 11, 3),        # and 3 (odd rows per block, last block short)
     # column tiles: rows wider than one block, halo columns each side
     (20, 1100, "f64", "channel", 8, 3),        # two tiles of 550
     (9, 2100, "f64", "periodic", 6, 0),        # three tiles, periodic wrap at both row ends
