@@ -72,3 +72,15 @@ def test_poiseuille_steady_profile_512x64():
     out = E.poiseuille_profile(512, 64, delta_p=1e-3, omega=1.5, steps=80000)
     assert out["max_rel_error"] < 0.02
     assert out["ratio"] == pytest.approx(1.0, abs=5e-3)
+
+
+def test_step_rates_command_reports_both_launch_modes():
+    """experiments.step_rates on a small Couette box: both rates present and
+    positive, buffer size as laid out (rows padded to 128 bytes)."""
+    from lattice_boltzmann_parallel_solver_amd import experiments as E
+
+    res = E.step_rates(64, 100, "couette", ("f32", "f64"), steps=20)
+    for st, elem, pitch in (("f32", 4, 128), ("f64", 8, 112)):
+        rates = res["glups"][st]
+        assert rates["1_per_launch"] > 0 and rates["2_per_launch"] > 0
+        assert res["state_buffer_bytes"][st] == 9 * 64 * pitch * elem
