@@ -35,12 +35,13 @@ Squaring semantics.  ``np.power(x, 2)`` is ``x*x`` in numpy 2.x, but numpy
 builds that route it through libm ``pow`` round differently in about 0.1 % of
 cases.  ``SQUARE`` selects the variant ("mul" by default, "pow" for libm).
 
-How each part is pinned (tests/test_oracle_*.py)
------------------------------------------------
+How each part is pinned (tests/test_oracle.py)
+---------------------------------------------
 * tables, streaming: closed form; streaming is a permutation (bit-exact).
 * density/velocity/equilibrium: numpy's own reductions on random inputs
-  (rounding order), invariants sum feq = rho and sum c feq = rho u, and golden
-  vectors written by ``tests/golden/make_golden.py`` from the reference.
+  (rounding order), invariants sum feq = rho and sum c feq = rho u, and the
+  reference's own result files, turned into fixtures by
+  ``tests/golden/make_reference_fixtures.py``.
 * BC primitives: the reference's known-answer tests
   (``tests/test_boundary_conditions.py:12-121``) plus the pressure constants
   re-derived by hand.

@@ -1,20 +1,26 @@
 """Batched step (one launch per step for many lattices, or two steps per
 launch with the batched sweep kernel) against individual lattices run one by
-one: bit for bit, including graph replays interleaved with eager work, odd
+one, bit for bit, including graph replays interleaved with eager work, odd
 step counts, mixed pending/running contexts, probes and jobs the sweep kernel
-does not take (the pressure fix)."""
+does not take (the pressure fix); and against the CPU oracle: FP64 batches bit
+for bit, FP32 batches within the stated tolerance of the FP64 oracle."""
 
 import numpy as np
 import pytest
 
+from oracle import lbm_oracle as O
+from oracle import lbm_oracle_native as N
+
 pytestmark = pytest.mark.gpu
 
+# FP32 deviation storage against the FP64 oracle over these runs (at most 42
+# steps of channels with u = 0.1): max absolute difference of f, rho and u.
+F32_TOL = {"f": 1e-6, "rho": 2e-6, "u": 2e-6}
 
-def make_jobs(storage="f64", seed=0, n=6, pressure=False, wide=False):
-    """Uneven channels plus a Couette and a periodic lattice."""
-    from lattice_boltzmann_parallel_solver_amd import Lattice
-    from lattice_boltzmann_parallel_solver_amd.boundary_utils import (couette_flow_boundary_conditions,
-                                                                      von_karman_boundary_conditions)
+
+def job_list(seed=0, n=6, pressure=False, wide=False):
+    """Uneven channels plus a Couette and a periodic lattice (and a Poiseuille
+    box): (lx, ly, kind, plate, rho, u, omega) per job."""
     from lattice_boltzmann_parallel_solver_amd.initial_values import (density_1_velocity_x_u0_velocity_y_0_initial,
                                                                       sinusoidal_velocity_x)
 
@@ -28,25 +34,71 @@ def make_jobs(storage="f64", seed=0, n=6, pressure=False, wide=False):
         d = max(2, int(round(rng.uniform(0.1, 0.4) * ly)))
         re = rng.uniform(40, 200)
         om = 1.0 / (3 * 0.1 * d / re + 0.5)
-        bc = von_karman_boundary_conditions(lx, ly, 1.0, 0.1, d, walls=True, y_offset=1)
         rho, u = density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), 0.1)
-        jobs.append((lx, ly, bc, rho, u, om))
-    rho = np.ones((33, 21))
-    jobs.append((33, 21, couette_flow_boundary_conditions(33, 21, 0.05, 1.0), rho, np.zeros((33, 21, 2)), 1.0))
+        jobs.append((lx, ly, "von_karman", d, rho, u, om))
+    jobs.append((33, 21, "couette", 0, np.ones((33, 21)), np.zeros((33, 21, 2)), 1.0))
     rho, u = sinusoidal_velocity_x((17, 64), 0.05)
-    jobs.append((17, 64, None, rho, u, 0.7))
+    jobs.append((17, 64, "periodic", 0, rho, u, 0.7))
     if pressure:
-        from lattice_boltzmann_parallel_solver_amd.boundary_utils import poiseuille_flow_boundary_conditions
+        jobs.append((30, 14, "poiseuille", 0, np.ones((30, 14)), np.zeros((30, 14, 2)), 1.4))
+    return jobs
 
-        rho = np.ones((30, 14))
-        jobs.append((30, 14, poiseuille_flow_boundary_conditions(30, 14, 1.0015 / 3, 0.9985 / 3), rho,
-                     np.zeros((30, 14, 2)), 1.4))
-    lats = []
-    for lx, ly, bc, rho, u, om in jobs:
-        lat = Lattice(lx, ly, boundary=bc, storage=storage)
+
+def _boundary(lx, ly, kind, d, oracle=False):
+    if oracle:
+        mod = O
+    else:
+        from lattice_boltzmann_parallel_solver_amd import boundary_utils as mod
+    if kind == "von_karman":
+        return mod.von_karman_boundary_conditions(lx, ly, 1.0, 0.1, d, walls=True, y_offset=1)
+    if kind == "couette":
+        return mod.couette_flow_boundary_conditions(lx, ly, 0.05, 1.0)
+    if kind == "poiseuille":
+        return mod.poiseuille_flow_boundary_conditions(lx, ly, 1.0015 / 3, 0.9985 / 3)
+    return None
+
+
+def make_jobs(storage="f64", seed=0, n=6, pressure=False, wide=False):
+    from lattice_boltzmann_parallel_solver_amd import Lattice
+
+    lats, oms = [], []
+    for lx, ly, kind, d, rho, u, om in job_list(seed, n, pressure, wide):
+        lat = Lattice(lx, ly, boundary=_boundary(lx, ly, kind, d), storage=storage)
         lat.set_equilibrium(rho, u)
         lats.append(lat)
-    return lats, [j[5] for j in jobs]
+        oms.append(om)
+    return lats, oms
+
+
+def oracle_runs(steps, seed=0, n=6, pressure=False, wide=False, probe=None, pre=None):
+    """The oracle's (f, rho, u, trace) of every job of job_list after `steps`
+    steps from its equilibrium; probe(lx, ly) gives the probe node, pre[k]
+    steps run before the probe starts recording (the trace covers the rest)."""
+    out = []
+    for k, (lx, ly, kind, d, rho, u, om) in enumerate(job_list(seed, n, pressure, wide)):
+        p = probe(lx, ly) if probe else None
+        first = pre[k] if pre else 0
+        f, r, v, trace = N.run(O.equilibrium(rho, u), rho, u, om, steps, _boundary(lx, ly, kind, d, oracle=True),
+                               probe=p)
+        out.append((f, r, v, trace[first:]))
+    return out
+
+
+def check_oracle(lats, want, storage, probes=False):
+    for lat, w in zip(lats, want):
+        got = lat.state()
+        for a, b, name in zip(got, w[:3], ("f", "rho", "u")):
+            if storage == "f64":
+                assert np.array_equal(a, b), (lat.lx, lat.ly, name, float(np.abs(a - b).max()))
+            else:
+                err = float(np.abs(a - b).max())
+                assert err <= F32_TOL[name], (lat.lx, lat.ly, name, err)
+        if probes:
+            smp = lat.probe()
+            if storage == "f64":
+                assert np.array_equal(smp, w[3]), (lat.lx, lat.ly)
+            else:
+                assert np.abs(smp - w[3]).max() <= F32_TOL["u"], (lat.lx, lat.ly)
 
 
 @pytest.mark.parametrize("v,graph", [(1, True), (2, True), (1, False)])
@@ -64,6 +116,7 @@ def test_batch_equals_individual_runs_fp64(v, graph):
             for a, c in zip(got, want):
                 assert np.array_equal(a, c)
     b.close()
+    check_oracle(lats, oracle_runs(42), "f64")
 
 
 def test_batch_with_running_and_pending_contexts_and_probes():
@@ -86,6 +139,9 @@ def test_batch_with_running_and_pending_contexts_and_probes():
         for a, c in zip(lat.state(), r.state()):
             assert np.array_equal(a, c)
     b.close()
+    want = [oracle_runs(26, seed=3, n=3, probe=lambda lx, ly: (lx // 2, ly // 2), pre=[5] + [0] * 4)[0]]
+    want += oracle_runs(21, seed=3, n=3, probe=lambda lx, ly: (lx // 2, ly // 2))[1:]
+    check_oracle(lats, want, "f64", probes=True)
 
 
 def test_batch_fp32_matches_individual():
@@ -100,6 +156,7 @@ def test_batch_fp32_matches_individual():
         for a, c in zip(lat.state(), r.state()):
             assert np.array_equal(a, c)
     b.close()
+    check_oracle(lats, oracle_runs(40, seed=5), "f32")
 
 
 def test_batch_errors():
@@ -141,6 +198,10 @@ def test_batched_sweep_equals_individual_single_steps(storage, wide):
         for a, c in zip(lat.state(), r.state()):
             assert np.array_equal(a, c)
     b.close()
+    if storage in ("f64", "f32"):
+        w = wide and storage != "f64"
+        check_oracle(lats, oracle_runs(35, seed=7, pressure=True, wide=w, probe=lambda lx, ly: (lx // 3, ly // 2)),
+                     storage, probes=True)
 
 
 def test_batch_switching_between_one_and_two_steps_per_launch_keeps_probes():
@@ -246,5 +307,15 @@ def test_large_batch_groups_on_side_streams_equal_individual_runs():
         for a, c in zip(lat.state(), r.state()):
             assert np.array_equal(a, c), (lat.lx, lat.ly)
         assert np.array_equal(lat.probe(), r.probe()), (lat.lx, lat.ly)
+    # against the FP64 oracle (17 steps)
+    want = []
+    for (lx, ly), om in zip(shapes, oms):
+        rho, u = O.density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), 0.1)
+        bc = O.von_karman_boundary_conditions(lx, ly, 1.0, 0.1, max(2, ly // 5), walls=True, y_offset=1)
+        want.append(N.run(O.equilibrium(rho, u), rho, u, om, 17, bc, probe=(3 * lx // 4, ly // 2)))
+    rho, u = O.density_1_velocity_0_initial((64, 32))
+    want.append(N.run(O.equilibrium(rho, u), rho, u, 1.5, 17,
+                      O.poiseuille_flow_boundary_conditions(64, 32, 1.0015 / 3, 0.9985 / 3), probe=(32, 16)))
+    check_oracle(lats, want, "f32", probes=True)
     for lat in lats + ref:
         lat.close()
