@@ -25,7 +25,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["lbm_kernels.hip", "lbm_capi.hip"]
-CXX_SOURCES = ["lbm_bc.cpp"]
+CXX_SOURCES = ["lbm_bc.cpp", "lbm_batch_tables.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall",

@@ -14,18 +14,30 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdarg>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
 #include <vector>
 
+#include "lbm_batch_tables.h"
 #include "lbm_launch.h"
 
 namespace lbm {
 namespace {
-thread_local std::string g_err;
+// The last error message of this thread, in a fixed buffer so that setting it
+// allocates nothing (the no-allocation entry point reports errors through it).
+thread_local char g_err[512];
+
+void set_error_fmt(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
 }
-void set_error(const std::string &msg) { g_err = msg; }
+}  // namespace
+void set_error(const std::string &msg) { set_error_fmt("%s", msg.c_str()); }
 }  // namespace lbm
 
 using namespace lbm;
@@ -151,7 +163,7 @@ hipMemcpyKind kind_out(int where) { return where == LBM_DEVICE ? hipMemcpyDevice
 
 extern "C" {
 
-const char *lbm_last_error(void) { return g_err.c_str(); }
+const char *lbm_last_error(void) { return g_err; }
 
 int lbm_version(void) { return 1; }
 
@@ -726,6 +738,14 @@ struct lbm_batch {
     std::vector<hipStream_t> lanes;
     std::vector<hipEvent_t> lane_done;
     hipEvent_t fork = nullptr;
+    // pinned host staging of the run's tables: the copies to the device read
+    // it asynchronously, and it is rewritten only once the batch stream and
+    // every side stream have drained (a run starts and ends that way)
+    void *h_stage = nullptr;
+    size_t h_stage_cap = 0;
+    TableImage image;
+    // set when this run has enqueued work on the contexts' state
+    bool dirty = false;
 };
 
 namespace {
@@ -738,9 +758,36 @@ void drop_graph(lbm_batch *b) {
     b->graph_blocks = 0;
 }
 
+// Wait for everything the batch has enqueued: its stream and every side stream.
+hipError_t quiesce(lbm_batch *b) {
+    hipError_t first = hipSuccess;
+    if (b->stream) first = hipStreamSynchronize(b->stream);
+    for (hipStream_t l : b->lanes) {
+        const hipError_t e = hipStreamSynchronize(l);
+        if (first == hipSuccess) first = e;
+    }
+    return first;
+}
+
+// Copy the packed tables into the batch's pinned staging buffer (grown as
+// needed); the caller has drained the batch's streams, so no copy reads it.
+hipError_t stage_tables(lbm_batch *b) {
+    const size_t bytes = b->image.bytes.size();
+    if (bytes > b->h_stage_cap) {
+        if (b->h_stage) (void)hipHostFree(b->h_stage);
+        b->h_stage = nullptr;
+        b->h_stage_cap = 0;
+        const hipError_t e = hipHostMalloc(&b->h_stage, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        b->h_stage_cap = bytes;
+    }
+    if (bytes) std::memcpy(b->h_stage, b->image.bytes.data(), bytes);
+    return hipSuccess;
+}
+
 void release_batch(lbm_batch *b) {
     (void)hipSetDevice(b->device);
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    (void)quiesce(b);
     drop_graph(b);
     (void)hipFree(b->d_jobs[0]);
     (void)hipFree(b->d_jobs[1]);
@@ -755,6 +802,7 @@ void release_batch(lbm_batch *b) {
     for (hipEvent_t e : b->lane_done) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
 }
 
 // Side streams of a batch, and the batch size (nodes) from which its groups
@@ -796,8 +844,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
         bool big = false;  // state buffers of 4 GiB or more
         size_t lds = 0;
         std::vector<int> members;
-        std::vector<uint32_t> block_job;  // index into members
-        size_t off_jobs[2] = {0, 0}, off_blocks = 0;
+        unsigned nblocks = 0;  // blocks of the group's launch
     };
     std::vector<Bucket> buckets;
     std::vector<int> fjobs;  // one-step jobs
@@ -823,16 +870,12 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
         }
         bk->members.push_back(j);
     }
-    // host tables: sweep jobs per bucket and pair parity, their block tables
-    std::vector<char> host;
-    auto put = [&](const void *data, size_t bytes) {
-        const size_t off = (host.size() + 255) & ~(size_t)255;
-        host.resize(off + bytes);
-        std::memcpy(host.data() + off, data, bytes);
-        return off;
-    };
-    for (Bucket &bk : buckets) {
-        std::vector<SweepJob> jobs[2];
+    // tables: sweep jobs per bucket and pair parity with their block tables,
+    // then the one-step jobs' records for the two steps of a pair
+    std::vector<SweepGroupTables> gt(buckets.size());
+    for (size_t bi = 0; bi < buckets.size(); bi++) {
+        Bucket &bk = buckets[bi];
+        SweepGroupTables &t = gt[bi];
         for (size_t m = 0; m < bk.members.size(); m++) {
             lbm_ctx *c = b->ctxs[bk.members[m]];
             // short rows (blocks of a few waves) need many blocks in flight:
@@ -841,8 +884,8 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             cfg.sweep_rows = bk.threads >= 512 ? std::max(b->sweep_rows, 32) : b->sweep_rows;
             const uint32_t rows = (uint32_t)sweep_rows_for(c->g, cfg, 0);
             const uint32_t nb = (c->g.lx + rows - 1) / rows;
-            const uint32_t block0 = (uint32_t)bk.block_job.size();
-            bk.block_job.insert(bk.block_job.end(), nb, (uint32_t)m);
+            const uint32_t block0 = (uint32_t)t.block_job.size();
+            t.block_job.insert(t.block_job.end(), nb, (uint32_t)m);
             bk.lds = std::max(bk.lds, sweep_lds_bytes(c->storage, c->g.ly, c->bc.view.chunk ? c->bc.view.nclasses : 0,
                                                       rows));
             for (int pp = 0; pp < 2; pp++) {
@@ -864,20 +907,17 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
                     d.probe_cap = c->probe.capacity;
                     d.probe_offset = c->steps - c->probe_base - 1 - b->step_host;
                 }
-                jobs[pp].push_back(d);
+                t.jobs[pp].push_back(d);
             }
         }
-        for (int pp = 0; pp < 2; pp++) bk.off_jobs[pp] = put(jobs[pp].data(), jobs[pp].size() * sizeof(SweepJob));
-        bk.off_blocks = put(bk.block_job.data(), bk.block_job.size() * sizeof(uint32_t));
+        bk.nblocks = (unsigned)t.block_job.size();
     }
-    // one-step jobs: tables for the two steps of a pair
-    std::vector<BatchJob> fj[2];
-    std::vector<uint32_t> fblocks;
+    StepTables st;
     for (size_t m = 0; m < fjobs.size(); m++) {
         lbm_ctx *c = b->ctxs[fjobs[m]];
         const uint32_t nb = (uint32_t)batch_job_blocks(c->g, b->v);
-        const uint32_t block0 = (uint32_t)fblocks.size();
-        fblocks.insert(fblocks.end(), nb, (uint32_t)m);
+        const uint32_t block0 = (uint32_t)st.block_job.size();
+        st.block_job.insert(st.block_job.end(), nb, (uint32_t)m);
         for (int p = 0; p < 2; p++) {
             BatchJob d;
             const int src = c->cur ^ p;
@@ -895,43 +935,41 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
                 d.probe_cap = c->probe.capacity;
                 d.probe_offset = c->steps - c->probe_base - 1 - b->step_host;
             }
-            fj[p].push_back(d);
+            st.jobs[p].push_back(d);
         }
     }
-    size_t off_f[2] = {0, 0}, off_fb = 0;
-    if (!fjobs.empty()) {
-        for (int p = 0; p < 2; p++) off_f[p] = put(fj[p].data(), fj[p].size() * sizeof(BatchJob));
-        off_fb = put(fblocks.data(), fblocks.size() * sizeof(uint32_t));
-    }
+    const unsigned nfblocks = (unsigned)st.block_job.size();
+    pack_batch_tables(gt, st, b->image);
+    const TableImage &img = b->image;
     for (const Bucket &bk : buckets)
         LBM_HIP_CHECK(prepare_sweep2_batch(b->storage, bk.v, bk.big, bk.lds), "lbm_batch_run: sweep kernel LDS");
     drop_graph(b);  // the one-step graph of an earlier run would hold stale tables
-    if (host.size() > b->sweep_cap) {
+    if (img.bytes.size() > b->sweep_cap) {
         (void)hipFree(b->d_sweep);
         b->d_sweep = nullptr;
         b->sweep_cap = 0;
-        LBM_HIP_CHECK(hipMalloc(&b->d_sweep, host.size()), "lbm_batch_run: sweep tables");
-        b->sweep_cap = host.size();
+        LBM_HIP_CHECK(hipMalloc(&b->d_sweep, img.bytes.size()), "lbm_batch_run: sweep tables");
+        b->sweep_cap = img.bytes.size();
     }
     char *dev = static_cast<char *>(b->d_sweep);
-    if (!host.empty())
-        LBM_HIP_CHECK(hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, b->stream),
-                      "lbm_batch_run: sweep tables");
+    LBM_HIP_CHECK(stage_tables(b), "lbm_batch_run: pinned staging of the tables");
+    LBM_HIP_CHECK(hipMemcpyAsync(dev, b->h_stage, img.bytes.size(), hipMemcpyHostToDevice, b->stream),
+                  "lbm_batch_run: sweep tables");
     // A pair, in order on the batch stream: the groups (they read the pair
     // counter d_step[1]), then the one-step jobs' two launches (they advance
     // d_step[0] themselves), then the pair counter.
     auto pair = [&](int pp) -> hipError_t {
         hipError_t e = hipSuccess;
-        for (const Bucket &bk : buckets) {
-            if (e != hipSuccess) break;
-            e = launch_sweep2_batch(b->storage, bk.v, bk.big, reinterpret_cast<const SweepJob *>(dev + bk.off_jobs[pp]),
-                                    reinterpret_cast<const uint32_t *>(dev + bk.off_blocks),
-                                    (unsigned)bk.block_job.size(), bk.threads, bk.lds, b->d_step + 1, 0, b->stream);
+        for (size_t i = 0; i < buckets.size() && e == hipSuccess; i++) {
+            const Bucket &bk = buckets[i];
+            e = launch_sweep2_batch(b->storage, bk.v, bk.big, reinterpret_cast<const SweepJob *>(dev + img.group_jobs[pp][i]),
+                                    reinterpret_cast<const uint32_t *>(dev + img.group_blocks[i]), bk.nblocks,
+                                    bk.threads, bk.lds, b->d_step + 1, 0, b->stream);
         }
         if (e == hipSuccess && !fjobs.empty())
             for (int p = 0; p < 2 && e == hipSuccess; p++)
-                e = launch_fused_batch(b->storage, b->v, reinterpret_cast<const BatchJob *>(dev + off_f[p]),
-                                       reinterpret_cast<const uint32_t *>(dev + off_fb), (unsigned)fblocks.size(),
+                e = launch_fused_batch(b->storage, b->v, reinterpret_cast<const BatchJob *>(dev + img.step_jobs[p]),
+                                       reinterpret_cast<const uint32_t *>(dev + img.step_blocks), nfblocks,
                                        b->d_step, b->d_done, b->stream);
         if (e == hipSuccess) e = launch_add_step(b->d_step + 1, 2, b->stream);
         if (e == hipSuccess && fjobs.empty()) e = launch_add_step(b->d_step, 2, b->stream);
@@ -977,17 +1015,19 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             const int pp = (int)(k & 1);
             for (size_t i = 0; i < buckets.size(); i++) {
                 const Bucket &bk = buckets[i];
-                LBM_HIP_CHECK(launch_sweep2_batch(b->storage, bk.v, bk.big, reinterpret_cast<const SweepJob *>(dev + bk.off_jobs[pp]),
-                                                  reinterpret_cast<const uint32_t *>(dev + bk.off_blocks),
-                                                  (unsigned)bk.block_job.size(), bk.threads, bk.lds, nullptr,
-                                                  b->step_host + 2 * k, b->lanes[lane_of[i]]),
+                LBM_HIP_CHECK(launch_sweep2_batch(b->storage, bk.v, bk.big,
+                                                  reinterpret_cast<const SweepJob *>(dev + img.group_jobs[pp][i]),
+                                                  reinterpret_cast<const uint32_t *>(dev + img.group_blocks[i]),
+                                                  bk.nblocks, bk.threads, bk.lds, nullptr, b->step_host + 2 * k,
+                                                  b->lanes[lane_of[i]]),
                               "batched sweep pair");
             }
             if (!fjobs.empty())
                 for (int p = 0; p < 2; p++)
-                    LBM_HIP_CHECK(launch_fused_batch(b->storage, b->v, reinterpret_cast<const BatchJob *>(dev + off_f[p]),
-                                                     reinterpret_cast<const uint32_t *>(dev + off_fb),
-                                                     (unsigned)fblocks.size(), b->d_step, b->d_done,
+                    LBM_HIP_CHECK(launch_fused_batch(b->storage, b->v,
+                                                     reinterpret_cast<const BatchJob *>(dev + img.step_jobs[p]),
+                                                     reinterpret_cast<const uint32_t *>(dev + img.step_blocks),
+                                                     nfblocks, b->d_step, b->d_done,
                                                      b->lanes[lane_of[buckets.size()]]),
                                   "batched one-step pair");
         }
@@ -1132,16 +1172,13 @@ int lbm_batch_set_temporal(lbm_batch *b, int steps_per_launch, int rows_per_bloc
     return LBM_OK;
 }
 
-int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
-    if (!b) {
-        set_error("lbm_batch_run: batch is NULL");
-        return LBM_ERR_ARG;
-    }
-    if (nsteps < 0) {
-        set_error("lbm_batch_run: nsteps must be >= 0");
-        return LBM_ERR_ARG;
-    }
-    LBM_HIP_CHECK(hipSetDevice(b->device), "lbm_batch_run");
+}  // extern "C"
+
+namespace {
+
+// The body of lbm_batch_run; it may return early on an error, with work of
+// this run still in flight (lbm_batch_run drains it).
+int batch_run_body(lbm_batch *b, int64_t nsteps) {
     const int n = (int)b->ctxs.size();
     for (int j = 0; j < n; j++) {
         lbm_ctx *c = b->ctxs[j];
@@ -1152,6 +1189,7 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
         LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "lbm_batch_run: context stream");
     }
     if (nsteps == 0) return LBM_OK;
+    b->dirty = true;  // from here on, an error leaves the contexts' states part-way
     // A context holding a loaded step input starts with its first collision,
     // which is one step; if any does, every other context takes one step on
     // its own too, so that the batched launches advance all of them alike.
@@ -1190,16 +1228,15 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
     }
 
     // job tables for the two step parities of this run
-    std::vector<BatchJob> jobs[2];
-    std::vector<uint32_t> block_job;
-    for (int p = 0; p < 2; p++) jobs[p].resize(n);
+    StepTables st;
+    for (int p = 0; p < 2; p++) st.jobs[p].resize(n);
     for (int j = 0; j < n; j++) {
         lbm_ctx *c = b->ctxs[j];
         const uint32_t nb = (uint32_t)batch_job_blocks(c->g, b->v);
-        const uint32_t block0 = (uint32_t)block_job.size();
-        block_job.insert(block_job.end(), nb, (uint32_t)j);
+        const uint32_t block0 = (uint32_t)st.block_job.size();
+        st.block_job.insert(st.block_job.end(), nb, (uint32_t)j);
         for (int p = 0; p < 2; p++) {
-            BatchJob &d = jobs[p][j];
+            BatchJob &d = st.jobs[p][j];
             const int src = c->cur ^ p;
             d.src = c->st.plane[src];
             d.dst = c->st.plane[src ^ 1];
@@ -1217,7 +1254,8 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
             }
         }
     }
-    const unsigned nblocks = (unsigned)block_job.size();
+    const unsigned nblocks = (unsigned)st.block_job.size();
+    pack_batch_tables({}, st, b->image);
     if (nblocks > b->block_cap) {
         drop_graph(b);
         (void)hipFree(b->d_block_job);
@@ -1225,11 +1263,15 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
         LBM_HIP_CHECK(hipMalloc(&b->d_block_job, sizeof(uint32_t) * nblocks), "lbm_batch_run: block table");
         b->block_cap = nblocks;
     }
-    LBM_HIP_CHECK(hipMemcpyAsync(b->d_block_job, block_job.data(), sizeof(uint32_t) * nblocks,
+    // through pinned staging into the fixed job and block buffers the
+    // captured step pair reads
+    LBM_HIP_CHECK(stage_tables(b), "lbm_batch_run: pinned staging of the tables");
+    const char *h = static_cast<const char *>(b->h_stage);
+    LBM_HIP_CHECK(hipMemcpyAsync(b->d_block_job, h + b->image.step_blocks, sizeof(uint32_t) * nblocks,
                                  hipMemcpyHostToDevice, b->stream), "lbm_batch_run: block table");
     for (int p = 0; p < 2; p++)
-        LBM_HIP_CHECK(hipMemcpyAsync(b->d_jobs[p], jobs[p].data(), sizeof(BatchJob) * n, hipMemcpyHostToDevice,
-                                     b->stream), "lbm_batch_run: job table");
+        LBM_HIP_CHECK(hipMemcpyAsync(b->d_jobs[p], h + b->image.step_jobs[p], sizeof(BatchJob) * n,
+                                     hipMemcpyHostToDevice, b->stream), "lbm_batch_run: job table");
     b->nblocks = nblocks;
 
     // step pairs replayed from the graph, then the odd one
@@ -1256,7 +1298,7 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
         for (int64_t k = 0; k < pairs; k++) LBM_HIP_CHECK(hipGraphLaunch(b->exec, b->stream), "graph launch");
         left -= 2 * pairs;
     }
-    for (int64_t k = 0; left > 0; k++, left--) {
+    for (; left > 0; left--) {
         const int p = (int)((nsteps - left) & 1);
         LBM_HIP_CHECK(launch_fused_batch(b->storage, b->v, b->d_jobs[p], b->d_block_job, nblocks, b->d_step,
                                          b->d_done, b->stream),
@@ -1274,8 +1316,42 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
                                             c->steps - c->probe_base - 1, b->stream),
                           "probe kernel");
     }
-    LBM_HIP_CHECK(hipStreamSynchronize(b->stream), "lbm_batch_run");
     return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
+    if (!b) {
+        set_error("lbm_batch_run: batch is NULL");
+        return LBM_ERR_ARG;
+    }
+    if (nsteps < 0) {
+        set_error("lbm_batch_run: nsteps must be >= 0");
+        return LBM_ERR_ARG;
+    }
+    LBM_HIP_CHECK(hipSetDevice(b->device), "lbm_batch_run");
+    // Nothing of an earlier run is in flight when its tables are rebuilt, and
+    // nothing of this run is when it returns, on success or on an error.
+    LBM_HIP_CHECK(quiesce(b), "lbm_batch_run: earlier work");
+    b->dirty = false;
+    int rc = batch_run_body(b, nsteps);
+    const hipError_t e = quiesce(b);
+    // a one-step run keeps its captured pair for the next run (it reads the
+    // batch's fixed job buffers); after an error nothing is kept
+    if (rc != LBM_OK || e != hipSuccess) drop_graph(b);
+    if (rc == LBM_OK && e != hipSuccess) {
+        set_error(std::string("lbm_batch_run: ") + hipGetErrorString(e));
+        rc = LBM_ERR_HIP;
+    }
+    if (rc != LBM_OK && b->dirty) {
+        // part of the run was enqueued: the contexts' states are unknown
+        for (lbm_ctx *c : b->ctxs) c->phase = kEmpty;
+        set_error(std::string(lbm_last_error()) + " (the batch's contexts lost their state; load it again)");
+    }
+    return rc;
 }
 
 // ---- stateless drop-ins ---------------------------------------------------
