@@ -25,9 +25,12 @@ batching   : aggregate MLUPS of 256 seeded uneven channels advanced together (on
              two-step launch per group of equal row width per step pair, the
              groups on side streams) versus the same channels solved one at a
              time, each at its autotuned launch shape.
-cpu_baseline: the oracle (numpy FP64) on a few steps of the same workload: P processes in
-             the reference's block decomposition (value), one process, and the C
-             restatement with OpenMP threads.
+f16        : the rate of FP16 deviation storage, with its measured error against
+             FP64 (it does not reproduce this channel's shedding).
+cpu_baseline: the oracle (numpy FP64) on windows of at least 10 s of the same
+             workload: P processes in the reference's block decomposition (value),
+             one process, and the C restatement with OpenMP threads; plus the
+             300x300 shear wave.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 """
@@ -136,18 +139,40 @@ def batching_leg(n=256, steps=1000, seed=0):
                         "profiles/batching_c4_256_channels.json"}
 
 
-def cpu_baseline(steps=2, mp_steps=6, c_steps=10):
-    """The reference's numpy path restated (the oracle, FP64) on this host's cores,
-    on a few steps of the same workload:
-
-    value          : P processes with the reference's MPI block decomposition
-                     (oracle/lbm_oracle_mp.py; P = host cores, at most 16)
-    single_process : the numpy oracle on one core
-    c_openmp       : the C restatement of the oracle with OpenMP threads
-
-    Runs before anything initialises the GPU (the multi-process leg forks)."""
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo where present)."""
     import platform
 
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(steps=4, mp_steps=30, c_steps=120, c1_numpy_steps=400, c1_steps=3000):
+    """The reference's numpy path restated (the oracle, FP64) on this host's
+    cores, each leg sized for a window of at least 10 s at the rates measured
+    on the GPU machines' hosts (channel: about 5 MLUPS on one core, 40 MLUPS on
+    16 processes, 180 MLUPS in C with 16 threads):
+
+    value          : P processes with the reference's MPI block decomposition
+                     (oracle/lbm_oracle_mp.py; P = host cores, at most 16),
+                     mp_steps steps of the 8192x2048 channel
+    single_process : the numpy oracle on one core, `steps` steps of the channel
+    c_openmp       : the C restatement of the oracle with OpenMP threads,
+                     c_steps steps of the channel
+    shear_wave_300 : the 300x300 shear wave (omega 1): the numpy oracle on one
+                     core over c1_numpy_steps steps, and the C restatement over
+                     the whole c1_steps-step run
+    MLUPS = nodes x steps / seconds / 1e6 (the reference's own timer,
+    experiments.py:765-772, divides nanoseconds by 10e9, which makes its stored
+    numbers 10x LUPS; here seconds are seconds).
+
+    Runs before anything initialises the GPU (the multi-process leg forks)."""
     import numpy as np
 
     from oracle import lbm_oracle as O
@@ -181,16 +206,57 @@ def cpu_baseline(steps=2, mp_steps=6, c_steps=10):
     assert np.isfinite(rho_c).all()
     dtc = (t2 - t1) - (t1 - t0)
     threads = int(NAT.lib().oracle_threads())
-    host = f"{platform.processor() or platform.machine()}, numpy {np.__version__}"
+    del f0, rho0, u0, rho_c
+
+    # the 300x300 shear wave
+    rs, us = O.sinusoidal_velocity_x((300, 300), 0.05)
+    fs = O.equilibrium(rs, us)
+    f, rho, u = O.step(fs, rs, us, 1.0)
+    t0 = time.perf_counter()
+    for _ in range(c1_numpy_steps):
+        f, rho, u = O.step(f, rho, u, 1.0)
+    dts = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    NAT.run(fs, rs, us, 1.0, c1_steps)
+    dtsc = time.perf_counter() - t0
+
+    host = f"{cpu_model()}, numpy {np.__version__}"
     return {"value": N * mp_steps / dtp / 1e6, "unit": "MLUPS", "cores": procs, "kind": "port",
             "sample": f"{mp_steps} steps (after 1 warm-up) of the {LX}x{LY} FP64 channel with the numpy oracle "
                       f"on {procs} processes in the reference's {grid[0]}x{grid[1]} block decomposition, "
                       f"halos through shared memory ({dtp:.1f} s); host {host}",
+            "cpu_model": cpu_model(), "numpy": np.__version__,
             "single_process": {"value": N * steps / dt1 / 1e6, "unit": "MLUPS", "cores": 1,
-                               "sample": f"{steps} steps after 1 warm-up ({dt1:.1f} s)"},
+                               "sample": f"{steps} steps of the channel after 1 warm-up ({dt1:.1f} s)"},
             "c_openmp": {"value": N * c_steps / dtc / 1e6, "unit": "MLUPS", "cores": threads,
-                         "sample": f"{c_steps} steps of the C restatement of the oracle, OpenMP "
-                                   f"({dtc:.1f} s, set-up subtracted by a 1-step run)"}}
+                         "sample": f"{c_steps} steps of the channel with the C restatement of the oracle, OpenMP "
+                                   f"({dtc:.1f} s, set-up subtracted by a 1-step run)"},
+            "shear_wave_300": {"numpy_mlups": 90000 * c1_numpy_steps / dts / 1e6, "numpy_cores": 1,
+                               "numpy_sample": f"{c1_numpy_steps} steps after 1 warm-up ({dts:.1f} s)",
+                               "numpy_projected_3000_steps_s": dts / c1_numpy_steps * 3000,
+                               "c_openmp_mlups": 90000 * c1_steps / dtsc / 1e6, "c_openmp_cores": threads,
+                               "c_openmp_sample": f"the whole {c1_steps}-step run ({dtsc:.1f} s)"}}
+
+
+def f16_error_note():
+    """The FP16 storage's measured error against the FP64 run of the same
+    channel (the committed precision study)."""
+    path = os.path.join(ROOT, "profiles", "precision_study_8192x2048.json")
+    try:
+        with open(path) as fh:
+            ps = json.load(fh)
+        cps = ps["checkpoints"]
+        out = {f"l2_u_rel_{k}": v["f16"]["l2_u"] for k, v in cps.items()}
+        out.update({f"l2_rho_rel_{k}": v["f16"]["l2_rho"] for k, v in cps.items()})
+        out["strouhal_uy_fp64"] = ps.get("strouhal", {}).get("f64", {}).get("strouhal_uy_interp")
+        out["strouhal_uy"] = None  # no shedding: no frequency to report
+    except (OSError, KeyError, ValueError):
+        out = {}
+    out["note"] = ("FP16 deviation storage does not reproduce this channel: the per-step change of a population "
+                   "(about 1e-6) is below the FP16 spacing of the deviations (1.5e-5), so the wake stagnates and "
+                   "no vortices shed; a rate for the storage, not a valid result for this configuration")
+    out["source"] = "profiles/precision_study_8192x2048.json"
+    return out
 
 
 def main():
@@ -199,6 +265,7 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "f64"])
+    ap.add_argument("--single-steps", type=int, default=200, help="steps of the one-step-per-launch leg")
     ap.add_argument("--fp64-steps", type=int, default=200)
     ap.add_argument("--f16-steps", type=int, default=200)
     ap.add_argument("--temporal", type=int, default=2, choices=[1, 2],
@@ -260,10 +327,12 @@ def main():
                 "steps_per_launch": spl_, "roofline_achieved_gbs": ach, "roofline_frac": ach / HBM_PEAK_GBS,
                 "bytes_per_node_step": BYTES_PER_NODE[storage] // spl_, "traffic": measured_traffic(storage, spl_)}
 
-    single = leg(args.storage, args.fp64_steps, 1) if spl == 2 else None
+    single = leg(args.storage, args.single_steps, 1) if spl == 2 else None
     fp64 = leg("f64", args.fp64_steps, args.temporal) if args.storage != "f64" else None
     fp64_single = leg("f64", args.fp64_steps, 1) if args.storage != "f64" and args.temporal == 2 else None
     f16 = leg("f16", args.f16_steps, args.temporal) if args.storage != "f16" else None
+    if f16 is not None:
+        f16["error_vs_fp64"] = f16_error_note()
 
     latency = None if args.no_serving else serving_leg()
     batching = None if args.no_batching else batching_leg()
