@@ -193,6 +193,22 @@ int lbm_batch_set_launch(lbm_batch *batch, int nodes_per_thread, int use_graph);
 int lbm_batch_set_temporal(lbm_batch *batch, int steps_per_launch, int rows_per_block);
 int lbm_batch_run(lbm_batch *batch, int64_t nsteps);
 
+/* ---- reductions that allocate nothing ----------------------------------- */
+/* Largest ncells lbm_moments_sum_noalloc accepts (174762 blocks of 256). */
+#define LBM_NOALLOC_MAX_CELLS 44739072
+/* FP64 sums of rho, rho*ux, rho*uy over `ncells` nodes of AoS FP64
+ * populations f (LBM_DEVICE, ncells*9 doubles), written to out[0..2]
+ * (LBM_DEVICE) asynchronously on `stream`.  The same kernels and reduction
+ * order as lbm_moments_sum on a context holding f as its step input, so the
+ * sums are identical.  No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops).
+ * It calls no hipMalloc/hipFree: the per-block partial sums go to a fixed
+ * 4 MiB device array that each GPU holds until the process ends, so ncells is
+ * at most LBM_NOALLOC_MAX_CELLS (LBM_ERR_ARG beyond).  All calls on one GPU
+ * share that array: calls that overlap on one GPU (other streams, other
+ * threads) race and must be ordered by the caller.  ncells = 0 writes zeros. */
+int lbm_moments_sum_noalloc(const double *f, int64_t ncells, double *out, void *stream);
+
 /* ---- stateless drop-ins on device arrays (AoS, C order) ----------------- */
 /* streaming(), lattice_boltzmann_method.py:140-159: periodic shift, bit-exact. */
 int lbm_stream(const void *f_in, void *f_out, int nx, int ny, int dtype, void *stream);

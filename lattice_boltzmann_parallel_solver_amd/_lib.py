@@ -107,6 +107,7 @@ _SIGNATURES = {
     "lbm_collide": (_I, [_P, _P, _P, _D, _P, _I64, _I, _P]),
     "lbm_bc_apply": (_I, [_P, _P, _P, _P, _P]),
     "lbm_bc_pressure": (_I, [_P, _P, _P, _P, _P]),
+    "lbm_moments_sum_noalloc": (_I, [_P, _I64, _P, _P]),
 }
 
 _lib = None

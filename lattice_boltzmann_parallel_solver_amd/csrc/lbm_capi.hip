@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1352,6 +1353,79 @@ int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
         set_error(std::string(lbm_last_error()) + " (the batch's contexts lost their state; load it again)");
     }
     return rc;
+}
+
+}  // extern "C"
+
+// ---- the reduction that allocates nothing --------------------------------
+// Block partial sums of lbm_moments_sum_noalloc: three doubles per 256-node
+// block of k_moments_sum_aos.  Each GPU holds its own copy from the first
+// load of this code object on it until the process ends.  In host code the
+// name stands for no device address: kernels get the per-device address that
+// hipGetSymbolAddress returns (noalloc_partials()).
+static __device__ __align__(256) double scratch[(4u << 20) / sizeof(double)];
+
+namespace {
+constexpr int kNoallocMaxDevices = 64;
+constexpr int64_t kNoallocBlocks = (int64_t)(sizeof(scratch) / (3 * sizeof(double)));
+static_assert(kNoallocBlocks * 256 == LBM_NOALLOC_MAX_CELLS, "LBM_NOALLOC_MAX_CELLS is the scratch's block count * 256");
+std::once_flag g_noalloc_once[kNoallocMaxDevices];
+void *g_noalloc_ptr[kNoallocMaxDevices];
+hipError_t g_noalloc_err[kNoallocMaxDevices];
+
+// The device address of `scratch` on device `dev` (current), looked up once
+// per device; null with the lookup's error in *err if it failed.
+double *noalloc_partials(int dev, hipError_t *err) {
+    std::call_once(g_noalloc_once[dev], [dev] {
+        void *p = nullptr;
+        g_noalloc_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_noalloc_ptr[dev] = g_noalloc_err[dev] == hipSuccess ? p : nullptr;
+    });
+    *err = g_noalloc_err[dev];
+    return static_cast<double *>(g_noalloc_ptr[dev]);
+}
+}  // namespace
+
+extern "C" {
+
+int lbm_moments_sum_noalloc(const double *f, int64_t ncells, double *out, void *stream) {
+    if (!out || ncells < 0 || (ncells > 0 && !f)) {
+        set_error_fmt("lbm_moments_sum_noalloc: need out != NULL, ncells >= 0 and f != NULL when ncells > 0");
+        return LBM_ERR_ARG;
+    }
+    if (ncells > LBM_NOALLOC_MAX_CELLS) {
+        set_error_fmt("lbm_moments_sum_noalloc: ncells = %lld exceeds LBM_NOALLOC_MAX_CELLS = %d (the fixed "
+                      "scratch holds %lld block sums)", (long long)ncells, LBM_NOALLOC_MAX_CELLS,
+                      (long long)kNoallocBlocks);
+        return LBM_ERR_ARG;
+    }
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error_fmt("lbm_moments_sum_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+        return LBM_ERR_HIP;
+    }
+    if (dev < 0 || dev >= kNoallocMaxDevices) {
+        set_error_fmt("lbm_moments_sum_noalloc: device %d is beyond the %d this library keeps scratch for", dev,
+                      kNoallocMaxDevices);
+        return LBM_ERR_ARG;
+    }
+    double *partials = noalloc_partials(dev, &e);
+    if (!partials) {
+        set_error_fmt("lbm_moments_sum_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        return LBM_ERR_HIP;
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    int nb = 0;
+    // every one of the nb partials is written before k_sum_partials reads it,
+    // so nothing left in the array by an earlier call is read
+    if (ncells > 0) e = launch_moments_sum_aos(f, ncells, partials, s, &nb);
+    if (e == hipSuccess) e = launch_sum_partials(partials, nb, out, s);
+    if (e != hipSuccess) {
+        set_error_fmt("lbm_moments_sum_noalloc: kernel launch: %s", hipGetErrorString(e));
+        return LBM_ERR_HIP;
+    }
+    return LBM_OK;
 }
 
 // ---- stateless drop-ins ---------------------------------------------------
