@@ -282,31 +282,48 @@ __device__ __forceinline__ void collide_node(const typename Io<S>::C v[9], doubl
     } else {
         // Contraction off and the fused multiply-adds written out, so that every
         // kernel rounds the reduced-precision collision the same way.
+        //   p_i = drho + rho (3 cu_i + 4.5 cu_i^2 - 1.5 |u|^2), f_eq,i - w_i = w_i p_i,
+        //   s_i = v_i + omega (w_i p_i - v_i) = (1 - omega) v_i + (omega w_i) p_i.
+        // Opposite directions share p's even part (c_opp = -c_i): per pair
+        // even = B + 4.5 rho cu^2 and odd = 3 rho cu give p_i = even + odd and
+        // p_opp = even - odd.
+        // 1/rho: the hardware reciprocal refined by one Newton step (within an
+        // ulp, like the correctly rounded division it replaces at a third of the
+        // instructions; the raw reciprocal alone doubled the FP32 error against
+        // FP64).
 #pragma clang fp contract(off)
         const float om = (float)omega;
+        const float om1 = 1.0f - om;
         const float drho = (((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]))) + v[8];
         const float rho = 1.0f + drho;
-        const float jx = ((v[1] - v[3]) + (v[5] - v[7])) + (v[8] - v[6]);
-        const float jy = ((v[2] - v[4]) + (v[5] - v[7])) + (v[6] - v[8]);
-        // FP16 storage: the hardware reciprocal (1 ulp) in place of a correctly
-        // rounded division, a tenth of the collision's instructions and far
-        // below the storage rounding (5-10 % faster).  FP32 keeps the division:
-        // the reciprocal doubled its error against FP64 (8192x2048 channel,
-        // 50k steps: |u| L2 2.2e-4 vs 8.4e-5) and gained no speed.
+        const float a57 = v[5] - v[7], b86 = v[8] - v[6];
+        const float jx = ((v[1] - v[3]) + a57) + b86;
+        const float jy = ((v[2] - v[4]) + a57) - b86;
         float inv = 0.0f;
         if (rho != 0.0f) {
-            if constexpr (sizeof(S) == 2) inv = __builtin_amdgcn_rcpf(rho);
-            else inv = 1.0f / rho;
+            const float r0 = __builtin_amdgcn_rcpf(rho);
+            inv = __builtin_fmaf(__builtin_fmaf(-rho, r0, 1.0f), r0, r0);
         }
         const float ux = jx * inv, uy = jy * inv;
-        const float base = -1.5f * __builtin_fmaf(ux, ux, uy * uy);
-        const float cu[9] = {0.0f, ux, uy, -ux, -uy, ux + uy, uy - ux, -(ux + uy), ux - uy};
+        const float usq = __builtin_fmaf(ux, ux, uy * uy);
+        const float B = __builtin_fmaf(-1.5f * rho, usq, drho);
+        const float r3 = 3.0f * rho, r45 = 4.5f * rho;
+        float pp[9];
+        pp[0] = B;
+        const float cu[4] = {ux, uy, ux + uy, uy - ux};  // directions 1, 2, 5, 6
+        constexpr int kPos[4] = {1, 2, 5, 6}, kNeg[4] = {3, 4, 7, 8};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float even = __builtin_fmaf(r45 * cu[k], cu[k], B);
+            const float odd = r3 * cu[k];
+            pp[kPos[k]] = even + odd;
+            pp[kNeg[k]] = even - odd;
+        }
 #pragma unroll
         for (int i = 0; i < 9; i++) {
-            // f_eq,i - w_i = w_i * (drho + rho*(3 cu + 4.5 cu^2 - 1.5 u^2))
-            const float t = __builtin_fmaf(cu[i], __builtin_fmaf(4.5f, cu[i], 3.0f), base);
-            o.e[i] = (float)kWd[i] * __builtin_fmaf(rho, t, drho);
-            o.s[i] = __builtin_fmaf(om, o.e[i] - v[i], v[i]);
+            const float wi = (float)kWd[i];
+            o.s[i] = __builtin_fmaf(om * wi, pp[i], om1 * v[i]);
+            o.e[i] = wi * pp[i];  // used by the pressure fix only
         }
         o.rho = 1.0 + (double)drho;
         o.jx = jx;
