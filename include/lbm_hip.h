@@ -191,6 +191,25 @@ int lbm_batch_set_launch(lbm_batch *batch, int nodes_per_thread, int use_graph);
  * more threads); the others keep the one-step batched kernel.  Results
  * are identical either way.  Default 1.  No counterpart in the reference. */
 int lbm_batch_set_temporal(lbm_batch *batch, int steps_per_launch, int rows_per_block);
+/* With two steps per launch, FP32/FP16 contexts of at most 1024 columns can
+ * be advanced by one packed launch per step pair: their row ranges,
+ * `rows_per_block` rows each, placed side by side in blocks of 16
+ * wavefronts, so lattices of every width share the launch and the pair is
+ * captured in a HIP graph on the batch's one stream.  packed = 1: one block
+ * per packed block; 2: a persistent grid of the blocks the GPU holds at once,
+ * each sweeping every grid-th packed block; 0 (default): one kernel per group
+ * of equal row width, on side streams in large batches (the fastest measured:
+ * 130.2 GLUPS on 256 uneven channels against 126.7 packed and 120.3
+ * persistent).  Results are identical in every mode.  No counterpart in the
+ * reference. */
+int lbm_batch_set_packing(lbm_batch *batch, int packed);
+
+/* ---- measurement ------------------------------------------------------ */
+/* Streaming device copy of `bytes` (a multiple of 16) from src to dst
+ * (LBM_DEVICE), 16 bytes per lane, asynchronous on `stream`: the practical
+ * HBM ceiling (read + write) the step kernels are measured against.  No
+ * counterpart in the reference. */
+int lbm_copy_probe(const void *src, void *dst, int64_t bytes, void *stream);
 int lbm_batch_run(lbm_batch *batch, int64_t nsteps);
 
 /* ---- reductions that allocate nothing ----------------------------------- */

@@ -99,6 +99,7 @@ _SIGNATURES = {
     "lbm_batch_destroy": (_I, [_P]),
     "lbm_batch_set_launch": (_I, [_P, _I, _I]),
     "lbm_batch_set_temporal": (_I, [_P, _I, _I]),
+    "lbm_batch_set_packing": (_I, [_P, _I]),
     "lbm_batch_run": (_I, [_P, _I64]),
     "lbm_stream": (_I, [_P, _P, _I, _I, _I, _P]),
     "lbm_equilibrium": (_I, [_P, _P, _P, _I64, _I, _P]),
@@ -108,6 +109,7 @@ _SIGNATURES = {
     "lbm_bc_apply": (_I, [_P, _P, _P, _P, _P]),
     "lbm_bc_pressure": (_I, [_P, _P, _P, _P, _P]),
     "lbm_moments_sum_noalloc": (_I, [_P, _I64, _P, _P]),
+    "lbm_copy_probe": (_I, [_P, _P, _I64, _P]),
 }
 
 _lib = None

@@ -21,6 +21,13 @@ struct SweepGroupTables {
     std::vector<uint32_t> block_job;  // launch block -> index into jobs[p]
 };
 
+// The records of the packed sweep launch (one per pair parity), its subs and blocks.
+struct PackedTables {
+    std::vector<SweepJob> jobs[2];
+    std::vector<PackedSub> subs;
+    std::vector<PackedBlock> blocks;
+};
+
 // The records of the jobs advanced by one-step launches.
 struct StepTables {
     std::vector<BatchJob> jobs[2];    // per step parity
@@ -33,6 +40,8 @@ struct TableImage {
     std::vector<char> bytes;
     std::vector<size_t> group_jobs[2];  // per group, per pair parity
     std::vector<size_t> group_blocks;   // per group
+    size_t packed_jobs[2] = {0, 0};
+    size_t packed_subs = 0, packed_blocks = 0;
     size_t step_jobs[2] = {0, 0};
     size_t step_blocks = 0;
 };
@@ -40,6 +49,22 @@ struct TableImage {
 constexpr size_t kTableAlign = 256;
 
 // Lay the tables out in `out` (its previous contents are replaced).
-void pack_batch_tables(const std::vector<SweepGroupTables> &groups, const StepTables &step, TableImage &out);
+void pack_batch_tables(const std::vector<SweepGroupTables> &groups, const PackedTables &packed, const StepTables &step,
+                       TableImage &out);
+
+// One row range to place in the packed sweep launch: `waves` waves of the
+// block and `lds` bytes of its LDS (16-byte multiple).
+struct PackItem {
+    uint32_t job = 0, xa = 0, rows = 0, waves = 0, lds = 0;
+};
+
+// Place the items in blocks of at most `block_waves` waves and `lds_budget`
+// bytes of LDS: items in order of decreasing width (then job, then row), each
+// into the open block with the fewest free waves that still holds it and has
+// the LDS, else a new block.  Consecutive row ranges of a job thus share a
+// block where they fit.  Items wider than a block, or with more LDS than the
+// budget, are refused (returns false).  Fills tables.subs and tables.blocks.
+bool plan_packed_blocks(const std::vector<PackItem> &items, uint32_t block_waves, uint32_t lds_budget,
+                        PackedTables &tables);
 
 }  // namespace lbm

@@ -731,6 +731,10 @@ struct lbm_batch {
     // two steps per launch for the jobs the sweep kernel supports
     int temporal = 1;
     int sweep_rows = 8;
+    // one packed launch for every job of one column per thread (FP32/FP16):
+    // 0 off (per-width groups, the faster on MI355X), 1 one block per packed
+    // block, 2 persistent blocks
+    int packed = 0;
     void *d_sweep = nullptr;  // sweep job tables and block tables of the last run
     size_t sweep_cap = 0;
     // side streams (default priority) that run the groups of a large batch
@@ -849,6 +853,8 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     };
     std::vector<Bucket> buckets;
     std::vector<int> fjobs;  // one-step jobs
+    std::vector<int> pjobs;  // jobs of the packed launch
+    const bool can_pack = b->packed != 0 && b->storage != LBM_STORE_F64;
     for (int j = 0; j < n; j++) {
         lbm_ctx *c = b->ctxs[j];
         if (!c->st.scratch || !sweep_batch_supported(c->storage, c->g, c->bc.view)) {
@@ -859,6 +865,10 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
         const int v = sh.v;
         const unsigned threads = sh.threads;
         const bool big = sweep_big(c->storage, c->g);
+        if (can_pack && v == 1 && !big && (c->g.ly + 63) / 64 <= packed_block_waves()) {
+            pjobs.push_back(j);
+            continue;
+        }
         Bucket *bk = nullptr;
         for (Bucket &x : buckets)
             if (x.v == v && x.threads == threads && x.big == big) bk = &x;
@@ -940,7 +950,62 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
         }
     }
     const unsigned nfblocks = (unsigned)st.block_job.size();
-    pack_batch_tables(gt, st, b->image);
+    // the packed launch: row ranges of `rows` rows, placed side by side in
+    // blocks of packed_block_waves() waves and at most half the CU's LDS (two
+    // blocks per CU), or more where one range needs it
+    PackedTables pt;
+    size_t plds = 0;
+    if (!pjobs.empty()) {
+        std::vector<PackItem> items;
+        uint32_t budget = 80u << 10;
+        for (size_t m = 0; m < pjobs.size(); m++) {
+            lbm_ctx *c = b->ctxs[pjobs[m]];
+            const uint32_t rows = (uint32_t)std::max(1, b->sweep_rows);
+            const uint32_t waves = (c->g.ly + 63) / 64;
+            const uint32_t ncl = c->bc.view.chunk ? c->bc.view.nclasses : 0;
+            for (uint32_t xa = 0; xa < c->g.lx; xa += rows) {
+                PackItem it;
+                it.job = (uint32_t)m;
+                it.xa = xa;
+                it.rows = std::min(rows, c->g.lx - xa);
+                it.waves = waves;
+                it.lds = packed_sub_lds(c->storage, c->g.ly, ncl, it.rows);
+                budget = std::max(budget, it.lds);
+                items.push_back(it);
+            }
+            for (int pp = 0; pp < 2; pp++) {
+                SweepJob d;
+                const int src = c->cur ^ pp;
+                d.src = c->st.plane[src];
+                d.dst = c->st.plane[src ^ 1];
+                d.side_prev = c->st.side[src];
+                d.side_next = c->st.side[src ^ 1];
+                d.scratch = c->st.scratch;
+                d.g = c->g;
+                d.bc = c->bc.view;
+                d.omega = b->omegas[pjobs[m]];
+                d.rows = rows;
+                if (c->probe.cell >= 0) {
+                    d.probe_cell = c->probe.cell;
+                    d.probe_ring = c->probe.ring;
+                    d.probe_cap = c->probe.capacity;
+                    d.probe_offset = c->steps - c->probe_base - 1 - b->step_host;
+                }
+                pt.jobs[pp].push_back(d);
+            }
+        }
+        if (!plan_packed_blocks(items, packed_block_waves(), budget, pt)) {
+            set_error("lbm_batch_run: packing the row ranges failed");
+            return LBM_ERR_ARG;
+        }
+        for (const PackedBlock &pb : pt.blocks) plds = std::max(plds, (size_t)pb.lds);
+        LBM_HIP_CHECK(prepare_sweep2_packed(b->storage, plds), "lbm_batch_run: packed kernel LDS");
+    }
+    const unsigned npblocks = (unsigned)pt.blocks.size();
+    unsigned pgrid = 0;
+    if (npblocks)
+        LBM_HIP_CHECK(packed_grid(b->storage, plds, npblocks, b->packed == 2, &pgrid), "lbm_batch_run: packed grid");
+    pack_batch_tables(gt, pt, st, b->image);
     const TableImage &img = b->image;
     for (const Bucket &bk : buckets)
         LBM_HIP_CHECK(prepare_sweep2_batch(b->storage, bk.v, bk.big, bk.lds), "lbm_batch_run: sweep kernel LDS");
@@ -961,6 +1026,11 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     // d_step[0] themselves), then the pair counter.
     auto pair = [&](int pp) -> hipError_t {
         hipError_t e = hipSuccess;
+        if (npblocks)
+            e = launch_sweep2_packed(b->storage, reinterpret_cast<const SweepJob *>(dev + img.packed_jobs[pp]),
+                                     reinterpret_cast<const PackedSub *>(dev + img.packed_subs),
+                                     reinterpret_cast<const PackedBlock *>(dev + img.packed_blocks), npblocks, pgrid,
+                                     plds, b->d_step + 1, 0, b->stream);
         for (size_t i = 0; i < buckets.size() && e == hipSuccess; i++) {
             const Bucket &bk = buckets[i];
             e = launch_sweep2_batch(b->storage, bk.v, bk.big, reinterpret_cast<const SweepJob *>(dev + img.group_jobs[pp][i]),
@@ -986,7 +1056,9 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     // one-step jobs keep their counter on their own stream.
     int64_t nodes = 0;
     for (lbm_ctx *c : b->ctxs) nodes += c->g.N;
-    const size_t groups = buckets.size() + (fjobs.empty() ? 0 : 1);
+    // groups: the buckets, then the packed launch, then the one-step jobs
+    const size_t gpacked = buckets.size(), gstep = buckets.size() + (npblocks ? 1 : 0);
+    const size_t groups = gstep + (fjobs.empty() ? 0 : 1);
     if (groups > 1 && nodes >= (int64_t)kBatchLaneNodes && pairs >= 1) {
         const size_t nl = std::min(groups, (size_t)kBatchLanes);
         LBM_HIP_CHECK(ensure_lanes(b, nl), "lbm_batch_run: side streams");
@@ -997,10 +1069,15 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             for (int j : buckets[i].members) g += b->ctxs[j]->g.N;
             load.push_back({g, i});
         }
+        if (npblocks) {
+            int64_t g = 0;
+            for (int j : pjobs) g += b->ctxs[j]->g.N;
+            load.push_back({g, gpacked});
+        }
         if (!fjobs.empty()) {
             int64_t g = 0;
             for (int j : fjobs) g += b->ctxs[j]->g.N;
-            load.push_back({g, buckets.size()});
+            load.push_back({g, gstep});
         }
         std::sort(load.begin(), load.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
         std::vector<int64_t> lane_load(nl, 0);
@@ -1023,13 +1100,20 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
                                                   b->lanes[lane_of[i]]),
                               "batched sweep pair");
             }
+            if (npblocks)
+                LBM_HIP_CHECK(launch_sweep2_packed(b->storage,
+                                                   reinterpret_cast<const SweepJob *>(dev + img.packed_jobs[pp]),
+                                                   reinterpret_cast<const PackedSub *>(dev + img.packed_subs),
+                                                   reinterpret_cast<const PackedBlock *>(dev + img.packed_blocks),
+                                                   npblocks, pgrid, plds, nullptr, b->step_host + 2 * k,
+                                                   b->lanes[lane_of[gpacked]]),
+                              "packed sweep pair");
             if (!fjobs.empty())
                 for (int p = 0; p < 2; p++)
                     LBM_HIP_CHECK(launch_fused_batch(b->storage, b->v,
                                                      reinterpret_cast<const BatchJob *>(dev + img.step_jobs[p]),
                                                      reinterpret_cast<const uint32_t *>(dev + img.step_blocks),
-                                                     nfblocks, b->d_step, b->d_done,
-                                                     b->lanes[lane_of[buckets.size()]]),
+                                                     nfblocks, b->d_step, b->d_done, b->lanes[lane_of[gstep]]),
                                   "batched one-step pair");
         }
         for (size_t l = 0; l < nl; l++) {
@@ -1062,6 +1146,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     b->step_host += 2 * pairs;
     for (const Bucket &bk : buckets)
         for (int j : bk.members) b->ctxs[j]->cur ^= (int)(pairs & 1);
+    for (int j : pjobs) b->ctxs[j]->cur ^= (int)(pairs & 1);
     for (int j = 0; j < n; j++) b->ctxs[j]->steps += 2 * pairs;
     if (nsteps & 1) {
         // the odd step, job by job
@@ -1256,7 +1341,7 @@ int batch_run_body(lbm_batch *b, int64_t nsteps) {
         }
     }
     const unsigned nblocks = (unsigned)st.block_job.size();
-    pack_batch_tables({}, st, b->image);
+    pack_batch_tables({}, PackedTables{}, st, b->image);
     if (nblocks > b->block_cap) {
         drop_graph(b);
         (void)hipFree(b->d_block_job);
@@ -1323,6 +1408,24 @@ int batch_run_body(lbm_batch *b, int64_t nsteps) {
 }  // namespace
 
 extern "C" {
+
+int lbm_batch_set_packing(lbm_batch *b, int packed) {
+    if (!b || packed < 0 || packed > 2) {
+        set_error("lbm_batch_set_packing: packed must be 0, 1 or 2");
+        return LBM_ERR_ARG;
+    }
+    b->packed = packed;
+    return LBM_OK;
+}
+
+int lbm_copy_probe(const void *src, void *dst, int64_t bytes, void *stream) {
+    if (bytes < 0 || bytes % 16 || (bytes > 0 && (!src || !dst))) {
+        set_error("lbm_copy_probe: need device buffers and bytes >= 0, a multiple of 16");
+        return LBM_ERR_ARG;
+    }
+    LBM_HIP_CHECK(launch_copy16(src, dst, bytes / 16, static_cast<hipStream_t>(stream)), "lbm_copy_probe");
+    return LBM_OK;
+}
 
 int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
     if (!b) {

@@ -23,6 +23,7 @@
 //   __half : deviations in FP16, FP32 arithmetic.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <type_traits>
 
 #include "lbm_launch.h"
@@ -53,12 +54,22 @@ constexpr double kWd[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9,
 template <class S>
 struct Io;
 
+// Pointers into global memory typed as such: loads and stores through them are
+// global accesses.  Through a generic pointer (one read from a job record
+// rather than passed as a kernel argument) the compiler emits flat accesses,
+// which count against the LDS wait counter too, so that every wait for LDS
+// data would also wait for the global loads in flight.
+#define LBM_GLOBAL __attribute__((address_space(1)))
+
+// The accessors take generic or global pointers (P).
 template <>
 struct Io<double> {
     using C = double;
     static constexpr bool kDev = false;
-    __device__ static C ld(const double *p) { return *p; }
-    __device__ static void st(double *p, C v, bool nt) {
+    template <class P>
+    __device__ static C ld(P p) { return *p; }
+    template <class P>
+    __device__ static void st(P p, C v, bool nt) {
         if (nt) __builtin_nontemporal_store(v, p);
         else *p = v;
     }
@@ -68,8 +79,10 @@ template <>
 struct Io<float> {
     using C = float;
     static constexpr bool kDev = true;
-    __device__ static C ld(const float *p) { return *p; }
-    __device__ static void st(float *p, C v, bool nt) {
+    template <class P>
+    __device__ static C ld(P p) { return *p; }
+    template <class P>
+    __device__ static void st(P p, C v, bool nt) {
         if (nt) __builtin_nontemporal_store(v, p);
         else *p = v;
     }
@@ -89,10 +102,19 @@ struct Io<__half> {
     using C = float;
     static constexpr bool kDev = true;
     __device__ static C ld(const __half *p) { return __half2float(*p); }
+    __device__ static C ld(const LBM_GLOBAL __half *p) {
+        return __half2float(__ushort_as_half(*(const LBM_GLOBAL unsigned short *)p));
+    }
     __device__ static void st(__half *p, C v, bool nt) {
         unsigned short bits = __half_as_ushort(f32_to_f16(v));
         if (nt) __builtin_nontemporal_store(bits, reinterpret_cast<unsigned short *>(p));
         else *reinterpret_cast<unsigned short *>(p) = bits;
+    }
+    __device__ static void st(LBM_GLOBAL __half *p, C v, bool nt) {
+        unsigned short bits = __half_as_ushort(f32_to_f16(v));
+        using U = LBM_GLOBAL unsigned short;
+        if (nt) __builtin_nontemporal_store(bits, (U *)p);
+        else *(U *)p = bits;
     }
 };
 
@@ -138,9 +160,8 @@ __device__ __forceinline__ uint32_t src_col(const Geom &g, uint32_t y, int cy) {
 }
 
 // Periodic pull: population i of node (x, y) comes from node (x - cx_i, y - cy_i).
-template <class S>
-__device__ __forceinline__ void pull(const S *__restrict__ src, const Geom &g, uint32_t x, uint32_t y,
-                                     typename Io<S>::C out[9]) {
+template <class S, class P>
+__device__ __forceinline__ void pull(P src, const Geom &g, uint32_t x, uint32_t y, typename Io<S>::C out[9]) {
 #pragma unroll
     for (int i = 0; i < 9; i++)
         out[i] = Io<S>::ld(src + pop(g, src_row(g, x, kCX[i]), i, src_col(g, y, kCY[i])));
@@ -157,6 +178,15 @@ __device__ __forceinline__ const S *at32(const S *base, uint32_t off) {
 template <class S>
 __device__ __forceinline__ S *at32(S *base, uint32_t off) {
     return reinterpret_cast<S *>(reinterpret_cast<char *>(base) + (uint32_t)(off * (uint32_t)sizeof(S)));
+}
+
+template <class S>
+__device__ __forceinline__ const LBM_GLOBAL S *at32(const LBM_GLOBAL S *base, uint32_t off) {
+    return (const LBM_GLOBAL S *)((const LBM_GLOBAL char *)base + (uint32_t)(off * (uint32_t)sizeof(S)));
+}
+template <class S>
+__device__ __forceinline__ LBM_GLOBAL S *at32(LBM_GLOBAL S *base, uint32_t off) {
+    return (LBM_GLOBAL S *)((LBM_GLOBAL char *)base + (uint32_t)(off * (uint32_t)sizeof(S)));
 }
 
 // Plane i of row r of a state buffer, from a 64-bit base (wave-uniform where
@@ -180,10 +210,9 @@ __device__ __forceinline__ S *pop32(S *base, uint32_t pitch, uint32_t r, int i, 
 }
 
 // pull() with 32-bit offsets: the three source rows are given.
-template <class S, bool kBig = false>
-__device__ __forceinline__ void pull32(const S *__restrict__ src, uint32_t pitch, uint32_t xm, uint32_t x,
-                                       uint32_t xp, uint32_t y, uint32_t ym, uint32_t yp,
-                                       typename Io<S>::C out[9]) {
+template <class S, bool kBig = false, class P>
+__device__ __forceinline__ void pull32(P src, uint32_t pitch, uint32_t xm, uint32_t x, uint32_t xp, uint32_t y,
+                                       uint32_t ym, uint32_t yp, typename Io<S>::C out[9]) {
 #pragma unroll
     for (int i = 0; i < 9; i++) {
         const uint32_t r = kCX[i] == 1 ? xm : (kCX[i] == -1 ? xp : x);
@@ -194,11 +223,9 @@ __device__ __forceinline__ void pull32(const S *__restrict__ src, uint32_t pitch
 
 // Apply the node's boundary class to the pulled populations; returns the
 // class flags (0 for plain fluid).
-template <class S>
-__device__ __forceinline__ uint32_t apply_class(const BcView &bc, const S *__restrict__ src,
-                                                const typename Io<S>::C *__restrict__ side_prev,
-                                                const Geom &g, uint32_t n, uint32_t x, uint32_t y,
-                                                typename Io<S>::C v[9]) {
+template <class S, class P, class Q>
+__device__ __forceinline__ uint32_t apply_class(const BcView &bc, P src, Q side_prev, const Geom &g, uint32_t n,
+                                                uint32_t x, uint32_t y, typename Io<S>::C v[9]) {
     using C = typename Io<S>::C;
     const uint32_t code = bc.codes[n];
     if (code == 0) return 0;
@@ -355,10 +382,9 @@ __device__ __forceinline__ void feq_rep(double rho_b, const NodeOut<S> &o, typen
 // (boundary_conditions.py:337-344): the node at x = lx-2 writes
 // E(rho_in, u) + (s - feq) into column 0 for directions D1, the node at x = 1
 // writes E(rho_out, u) + (s - feq) into column lx-1 for directions D2.
-template <class S>
-__device__ __forceinline__ void store_node(S *__restrict__ dst, typename Io<S>::C *__restrict__ side_next,
-                                           const Geom &g, const BcView &bc, uint32_t x, uint32_t y,
-                                           uint32_t flags, const typename Io<S>::C v[9],
+template <class S, class P, class Q>
+__device__ __forceinline__ void store_node(P dst, Q side_next, const Geom &g, const BcView &bc, uint32_t x,
+                                           uint32_t y, uint32_t flags, const typename Io<S>::C v[9],
                                            const NodeOut<S> &o, bool nt) {
     using C = typename Io<S>::C;
     const uint32_t skip = flags >> kSkipShift;
@@ -681,10 +707,11 @@ __global__ __launch_bounds__(kBlock) void k_fused_batch(const BatchJob *__restri
     using C = typename Io<S>::C;
     const BatchJob &d = jobs[block_job[blockIdx.x]];
     const Geom &g = d.g;
-    const S *src = static_cast<const S *>(d.src);
-    S *dst = static_cast<S *>(d.dst);
-    const C *side_prev = static_cast<const C *>(d.side_prev);
-    C *side_next = static_cast<C *>(d.side_next);
+    // global pointers (read from the job record the compiler cannot tell)
+    const LBM_GLOBAL S *src = (const LBM_GLOBAL S *)d.src;
+    LBM_GLOBAL S *dst = (LBM_GLOBAL S *)d.dst;
+    const LBM_GLOBAL C *side_prev = (const LBM_GLOBAL C *)d.side_prev;
+    LBM_GLOBAL C *side_next = (LBM_GLOBAL C *)d.side_next;
     const int64_t s_now = *step;
     const uint32_t lb = blockIdx.x - d.block0;
 #pragma unroll
@@ -705,7 +732,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_batch(const BatchJob *__restri
             if ((int64_t)n == d.probe_cell) {
                 const int64_t slot = s_now + d.probe_offset;
                 if (slot >= 0) {
-                    double *out = d.probe_ring + 3 * (slot % d.probe_cap);
+                    LBM_GLOBAL double *out = (LBM_GLOBAL double *)d.probe_ring + 3 * (slot % d.probe_cap);
                     out[0] = o.rho;
                     out[1] = (double)o.ux;
                     out[2] = (double)o.uy;
@@ -822,10 +849,9 @@ __host__ __device__ inline SweepLds sweep_lds_layout(uint32_t cols, uint32_t ncl
 // pull32() for a node with a boundary class: directions the class reflects
 // load the node's own opposite population (the reflection's source) in place
 // of the upstream one, so that no load is needed later.
-template <class S, bool kBig>
-__device__ __forceinline__ void pull32_bc(const S *__restrict__ src, uint32_t pitch, uint32_t xm, uint32_t x,
-                                          uint32_t xp, uint32_t y, uint32_t ym, uint32_t yp, uint32_t w,
-                                          typename Io<S>::C out[9]) {
+template <class S, bool kBig, class P>
+__device__ __forceinline__ void pull32_bc(P src, uint32_t pitch, uint32_t xm, uint32_t x, uint32_t xp, uint32_t y,
+                                          uint32_t ym, uint32_t yp, uint32_t w, typename Io<S>::C out[9]) {
 #pragma unroll
     for (int i = 0; i < 9; i++) {
         const uint32_t r = kCX[i] == 1 ? xm : (kCX[i] == -1 ? xp : x);
@@ -845,20 +871,41 @@ struct Flag {
     static constexpr bool value = B;
 };
 
+// The threads that sweep one row range: the whole block, or in a packed block
+// one run of its waves (tid counts from the run's first thread; T threads;
+// its LDS area from smem), and the row iterations every wave of the block
+// runs (packed blocks: the longest range's rows + 2).
+struct SweepLanes {
+    uint32_t tid, T;
+    unsigned char *smem;
+    uint32_t r_block;
+};
+
 // The rows [xa, xa + p.rows) of one lattice (clipped at lx), two steps.
 // With kTiled the block owns only output columns [cy0, cy0 + W) of them, cy0 =
 // tile * tw: step 1 also runs on one halo column each side (periodic in y),
 // whose values step 2 needs; the halo columns' step-2 results are not kept.
 // LDS and lane columns are then local: local column l is global column
 // cy0 - 1 + l (mod ly).  kBig: state buffers of 4 GiB and more (64-bit row
-// bases).
-template <class S, int V, bool kProbe, bool kTiled, bool kBig>
-__device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa, uint32_t tile) {
+// bases).  kPacked: the lanes share their block with other row ranges, so
+// every barrier is block-wide and runs the same number of times on every
+// wave: the set-up barrier always, then ln.r_block + 2 row iterations (those
+// past this range's last row do nothing but the barrier).
+template <class S, int V, bool kProbe, bool kTiled, bool kBig, bool kPacked = false>
+__device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa, uint32_t tile, SweepLanes ln) {
     using C = typename Io<S>::C;
-    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned char *smem = ln.smem;
     const Geom &g = p.g;
     const uint32_t ly = g.ly, lx = g.lx, pitch = g.pitch;
-    const uint32_t T = blockDim.x;
+    const uint32_t T = ln.T, tid = ln.tid;
+    // the buffers as global pointers (the batched kernels read them from job
+    // records, where the compiler cannot tell their address space)
+    using GS = LBM_GLOBAL S;
+    const GS *gsrc = (const GS *)p.src;
+    GS *gdst = (GS *)p.dst;
+    GS *gscr = (GS *)p.scratch;
+    LBM_GLOBAL C *gside = (LBM_GLOBAL C *)p.side_next;
+    const LBM_GLOBAL uint16_t *gcodes = (const LBM_GLOBAL uint16_t *)p.bc.codes;
     const uint32_t R = min(p.rows, lx - xa);
     const uint32_t cy0 = kTiled ? tile * p.tw : 0;  // first output column
     const uint32_t W = kTiled ? min(p.tw, ly - cy0) : ly;  // output columns
@@ -885,24 +932,24 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
     };
     if (has_bc) {
         const uint32_t words = p.bc.nclasses * (uint32_t)(sizeof(ClassEntry) / 4);
-        for (uint32_t i = threadIdx.x; i < words; i += T)
-            reinterpret_cast<uint32_t *>(cls)[i] = reinterpret_cast<const uint32_t *>(p.bc.classes)[i];
+        for (uint32_t i = tid; i < words; i += T)
+            reinterpret_cast<uint32_t *>(cls)[i] = ((const LBM_GLOBAL uint32_t *)p.bc.classes)[i];
         // step 1 reads the outlet values of the previous launch from LDS: only
         // blocks whose step-1 rows include lx-1 (the last block, and the first
         // through the periodic row xa-1) need them
         if ((p.bc.flags & kProgOutlet) && (xa == 0 || xa + R + 1 >= lx)) {
-            for (uint32_t i = threadIdx.x; i < 3 * lw; i += T) {
+            for (uint32_t i = tid; i < 3 * lw; i += T) {
                 if constexpr (kTiled) {
                     const uint32_t l = i / 3;
-                    side_in[i] = p.side_prev[gcol(l) * 3 + (i - 3 * l)];
+                    side_in[i] = ((const LBM_GLOBAL C *)p.side_prev)[gcol(l) * 3 + (i - 3 * l)];
                 } else {
-                    side_in[i] = p.side_prev[i];
+                    side_in[i] = ((const LBM_GLOBAL C *)p.side_prev)[i];
                 }
             }
         }
-        for (uint32_t i = threadIdx.x; i < (R + 2) * nw; i += T) {
+        for (uint32_t i = tid; i < (R + 2) * nw; i += T) {
             const uint32_t rr = i / nw, j = i - rr * nw;
-            const uint8_t *wr = p.bc.wave + (size_t)row_at(rr) * g.wpr;
+            const LBM_GLOBAL uint8_t *wr = (const LBM_GLOBAL uint8_t *)p.bc.wave + (size_t)row_at(rr) * g.wpr;
             if constexpr (kTiled) {
                 // local wave j spans global columns a .. b (unwrapped, a >= -1,
                 // b <= ly): the global waves of both ends, and of both row
@@ -918,8 +965,9 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
                 wflag[i] = wr[j];
             }
         }
-        __syncthreads();
+        if constexpr (!kPacked) __syncthreads();
     }
+    if constexpr (kPacked) __syncthreads();
     // Every lane is active: lanes past the last column repeat column ly-1 with
     // the same inputs, boundary class and results (their stores write the same
     // values to the same places).  With no lane-dependent branch around memory
@@ -930,11 +978,11 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
     uint32_t ys[V], ym[V], yp[V], jw[V], ls[V], lm[V], lp[V];
 #pragma unroll
     for (int k = 0; k < V; k++) {
-        const uint32_t l = min(threadIdx.x + k * T, lw - 1);
+        const uint32_t l = min(tid + k * T, lw - 1);
         ys[k] = kTiled ? gcol(l) : l;
         ym[k] = ys[k] == 0 ? ly - 1 : ys[k] - 1;
         yp[k] = ys[k] + 1 >= ly ? 0 : ys[k] + 1;
-        jw[k] = min((threadIdx.x >> 6) + k * (T >> 6), nw - 1);
+        jw[k] = min((tid >> 6) + k * (T >> 6), nw - 1);
         ls[k] = l;
         lm[k] = l == 0 ? 0 : l - 1;
         lp[k] = min(l + 1, lw - 1);
@@ -945,7 +993,7 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
     // the lane's column is an output column of this block (tiles: not a halo
     // or clamped lane)
     auto owned = [&](int k) {
-        const uint32_t lu = threadIdx.x + k * T;
+        const uint32_t lu = tid + k * T;
         return !kTiled || (lu >= 1 && lu <= W);
     };
     // the wave holding column k has boundary nodes in row iteration rr (wave-uniform)
@@ -955,7 +1003,7 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
     };
     auto load_code = [&](uint32_t rr, int k) -> uint32_t {
         if (!special(rr, k)) return 0;
-        return p.bc.codes[row_at(rr) * ly + ys[k]];
+        return gcodes[row_at(rr) * ly + ys[k]];
     };
     // the pulls of row iteration rr for the lane's columns k0 <= k < k1
     auto pull_row = [&](uint32_t rr, const uint32_t code[V], C out[V][9], auto k0_tag, auto k1_tag) {
@@ -965,9 +1013,9 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
         for (int k = k0; k < k1; k++) {
             if (special(rr, k)) {
                 const uint32_t w = code[k] ? cls[code[k]].src : 0u;
-                pull32_bc<S, kBig>(p.src, pitch, xm, xr, xp, ys[k], ym[k], yp[k], w, out[k]);
+                pull32_bc<S, kBig>(gsrc, pitch, xm, xr, xp, ys[k], ym[k], yp[k], w, out[k]);
             } else {
-                pull32<S, kBig>(p.src, pitch, xm, xr, xp, ys[k], ym[k], yp[k], out[k]);
+                pull32<S, kBig>(gsrc, pitch, xm, xr, xp, ys[k], ym[k], yp[k], out[k]);
             }
         }
     };
@@ -1051,9 +1099,10 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
             }
             if constexpr (kProbe) {
                 if (p.probe1 && it >= 1 && it <= R && owned(k) && (int64_t)(x * ly + y) == p.probe_cell) {
-                    p.probe1[0] = o.rho;
-                    p.probe1[1] = (double)o.ux;
-                    p.probe1[2] = (double)o.uy;
+                    LBM_GLOBAL double *pr = (LBM_GLOBAL double *)p.probe1;
+                    pr[0] = o.rho;
+                    pr[1] = (double)o.ux;
+                    pr[2] = (double)o.uy;
                 }
             }
             // Only the column's own lane publishes it.  A lane past the last
@@ -1063,7 +1112,7 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
             // lanes write the spare column at the end of each plane (or, in a
             // narrower last tile, columns past its lw that nobody reads) instead.
             {
-                const uint32_t col = min(threadIdx.x + k * T, cw);
+                const uint32_t col = min(tid + k * T, cw);
 #pragma unroll
                 for (int j = 0; j < 6; j++) lds_put<S>(exb + j * cs + col, o.s[kYM[j]]);
             }
@@ -1132,29 +1181,30 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
                 if constexpr (kTiled) {
                     // output and padding columns go to the output; halo lanes,
                     // and clamped lanes over other tiles' columns, to scratch
-                    const uint32_t lu = threadIdx.x + k * T;
+                    const uint32_t lu = tid + k * T;
                     const uint32_t vc = cy0 + lu - 1;  // lu = 0 only with the left halo
                     const bool keep = lu >= 1 && (lu <= W || (vc >= ly && vc < pitch));
-                    S *base = keep ? p.dst + (size_t)r * 9 * pitch + vc : p.scratch + (size_t)k * 9 * T + threadIdx.x;
+                    GS *base = keep ? gdst + (size_t)r * 9 * pitch + vc : gscr + (size_t)k * 9 * T + tid;
                     const uint32_t stride = keep ? pitch : T;
 #pragma unroll
                     for (int q = 0; q < 9; q++) Io<S>::st(base + (size_t)q * stride, o2.s[q], p.nt != 0);
                 } else {
-                    const uint32_t yst = min(threadIdx.x + k * T, pitch - 1);
+                    const uint32_t yst = min(tid + k * T, pitch - 1);
 #pragma unroll
                     for (int q = 0; q < 9; q++)
-                        Io<S>::st(pop32<kBig>(p.dst, pitch, r, q, yst), o2.s[q], p.nt != 0);
+                        Io<S>::st(pop32<kBig>(gdst, pitch, r, q, yst), o2.s[q], p.nt != 0);
                 }
                 if ((flags2 & kFlagOutletSide) && owned(k)) {
-                    p.side_next[(size_t)y * 3 + 0] = v2[3];
-                    p.side_next[(size_t)y * 3 + 1] = v2[6];
-                    p.side_next[(size_t)y * 3 + 2] = v2[7];
+                    gside[(size_t)y * 3 + 0] = v2[3];
+                    gside[(size_t)y * 3 + 1] = v2[6];
+                    gside[(size_t)y * 3 + 2] = v2[7];
                 }
                 if constexpr (kProbe) {
                     if (p.probe2 && owned(k) && (int64_t)(r * ly + y) == p.probe_cell) {
-                        p.probe2[0] = o2.rho;
-                        p.probe2[1] = (double)o2.ux;
-                        p.probe2[2] = (double)o2.uy;
+                        LBM_GLOBAL double *pr = (LBM_GLOBAL double *)p.probe2;
+                        pr[0] = o2.rho;
+                        pr[1] = (double)o2.ux;
+                        pr[2] = (double)o2.uy;
                     }
                 }
             }
@@ -1181,22 +1231,26 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
 #pragma unroll
     for (int k = 0; k < V; k++)
 #pragma unroll
-        for (int q = 0; q < 9; q++) Io<S>::st(at32(p.scratch, (k * 9 + q) * T + threadIdx.x), (C)0, false);
+        for (int q = 0; q < 9; q++) Io<S>::st(at32(gscr, (k * 9 + q) * T + tid), (C)0, false);
     // steady state: prefetch, step 1, step 2
     for (uint32_t it = 2; it <= R; it++) body(it, Flag<true>{}, Flag<true>{});
     // epilogue: the last step-1 row (no prefetch) and the last output row
     body(R + 1, Flag<false>{}, Flag<true>{});
+    if constexpr (kPacked)
+        for (uint32_t it = R + 2; it < ln.r_block + 2; it++) __syncthreads();
 }
 
 template <class S, int V, bool kProbe, bool kTiled, bool kBig>
 __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
+    extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t blk = p.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const SweepLanes ln{threadIdx.x, blockDim.x, smem, 0};
     if constexpr (kTiled) {
         // the tiles of one row range on consecutive blocks (one XCD's L2)
         const uint32_t rb = blk / p.ntiles;
-        sweep_block<S, V, kProbe, true, kBig>(p, rb * p.rows, blk - rb * p.ntiles);
+        sweep_block<S, V, kProbe, true, kBig>(p, rb * p.rows, blk - rb * p.ntiles, ln);
     } else {
-        sweep_block<S, V, kProbe, false, kBig>(p, blk * p.rows, 0);
+        sweep_block<S, V, kProbe, false, kBig>(p, blk * p.rows, 0, ln);
     }
 }
 
@@ -1208,6 +1262,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2_batch(const SweepJob *
                                                                 const uint32_t *__restrict__ block_job,
                                                                 const int64_t *__restrict__ step, int64_t step_now) {
     using C = typename Io<S>::C;
+    extern __shared__ __align__(16) unsigned char smem[];
     // consecutive blocks (neighbouring row ranges of a job) on one XCD, so the
     // halo rows they both read can hit that XCD's L2
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -1233,7 +1288,74 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2_batch(const SweepJob *
     p.scratch = static_cast<S *>(j.scratch);
     p.ntiles = 1;
     p.tw = p.lw = j.g.ly;
-    sweep_block<S, V, true, false, kBig>(p, (blk - j.block0) * j.rows, 0);
+    sweep_block<S, V, true, false, kBig>(p, (blk - j.block0) * j.rows, 0, SweepLanes{threadIdx.x, blockDim.x, smem, 0});
+}
+
+// Two steps of many row ranges in one persistent launch: a grid of resident
+// blocks, block b sweeping the subs of packed blocks b, b + G, b + 2G, ...,
+// each packed block's subs side by side on their own waves and LDS areas (one
+// column per thread, state buffers below 4 GiB).  Waves that belong to no sub
+// of a packed block take part in its barriers only.
+constexpr int kPackedWaves = 16;
+
+template <class S>
+__device__ __forceinline__ void packed_item(const SweepJob *__restrict__ jobs, const PackedSub *__restrict__ subs,
+                                            const PackedBlock &pb, const int64_t *__restrict__ step,
+                                            int64_t step_now, unsigned char *smem) {
+    using C = typename Io<S>::C;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t mine = pb.nsub;
+    for (uint32_t k = 0; k < pb.nsub; k++) {
+        const PackedSub &q = subs[pb.sub0 + k];
+        if (wave >= q.wave0 && wave < q.wave0 + q.waves) mine = k;
+    }
+    mine = __builtin_amdgcn_readfirstlane(mine);
+    if (mine == pb.nsub) {
+        // the set-up barrier and one per row iteration, as the swept waves
+        for (uint32_t i = 0; i < pb.r_block + 3; i++) __syncthreads();
+        return;
+    }
+    const PackedSub q = subs[pb.sub0 + mine];
+    const SweepJob &j = jobs[q.job];
+    SweepParams<S> p;
+    p.src = static_cast<const S *>(j.src);
+    p.dst = static_cast<S *>(j.dst);
+    p.side_prev = static_cast<const C *>(j.side_prev);
+    p.side_next = static_cast<C *>(j.side_next);
+    p.g = j.g;
+    p.bc = j.bc;
+    p.omega = j.omega;
+    p.probe_cell = j.probe_cell;
+    p.probe1 = p.probe2 = nullptr;
+    if (j.probe_cell >= 0) {
+        const int64_t slot = (step ? *step : step_now) + j.probe_offset;
+        if (slot >= 0) p.probe1 = j.probe_ring + 3 * (slot % j.probe_cap);
+        if (slot + 1 >= 0) p.probe2 = j.probe_ring + 3 * ((slot + 1) % j.probe_cap);
+    }
+    p.rows = q.rows;
+    p.nt = 0;
+    p.xcd = 1;
+    p.scratch = static_cast<S *>(j.scratch);
+    p.ntiles = 1;
+    p.tw = p.lw = j.g.ly;
+    const SweepLanes ln{threadIdx.x - q.wave0 * 64, q.waves * 64, smem + q.lds_off, pb.r_block};
+    sweep_block<S, 1, true, false, false, true>(p, q.xa, 0, ln);
+}
+
+template <class S>
+__global__ __launch_bounds__(kPackedWaves * 64, 8) void k_sweep2_packed(const SweepJob *__restrict__ jobs,
+                                                                         const PackedSub *__restrict__ subs,
+                                                                         const PackedBlock *__restrict__ blocks,
+                                                                         uint32_t nblocks,
+                                                                         const int64_t *__restrict__ step,
+                                                                         int64_t step_now) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        // the previous packed block's last LDS reads come after its last
+        // barrier; this one's set-up writes the same LDS
+        if (b != blockIdx.x) __syncthreads();
+        packed_item<S>(jobs, subs, blocks[b], step, step_now, smem);
+    }
 }
 
 __global__ void k_add_step(int64_t *step, int64_t k) { *step += k; }
@@ -1795,6 +1917,78 @@ hipError_t launch_sweep2_batch(int storage, int v, bool big, const SweepJob *job
     void *args[] = {&jobs, &block_job, &step, &step_now};
     hipError_t e = hipLaunchKernel(fn, dim3(nblocks), dim3(threads), args, lds, s);
     return e != hipSuccess ? e : hipGetLastError();
+}
+
+namespace {
+// FP32 and FP16 only: at the 64 registers that let two 16-wave blocks share a
+// CU, the FP64 instance spills about 120 of them.
+const void *sweep_packed_fn(int storage) {
+    switch (storage) {
+        case LBM_STORE_F32: return reinterpret_cast<const void *>(k_sweep2_packed<float>);
+        case LBM_STORE_F16: return reinterpret_cast<const void *>(k_sweep2_packed<__half>);
+    }
+    return nullptr;
+}
+}  // namespace
+
+uint32_t packed_block_waves() { return kPackedWaves; }
+
+uint32_t packed_sub_lds(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows) {
+    size_t b = 0;
+    switch (storage) {
+        case LBM_STORE_F64: b = sweep_lds_layout<double>(ly, nclasses, rows).total; break;
+        case LBM_STORE_F32: b = sweep_lds_layout<float>(ly, nclasses, rows).total; break;
+        case LBM_STORE_F16: b = sweep_lds_layout<__half>(ly, nclasses, rows).total; break;
+    }
+    return (uint32_t)align16(b);
+}
+
+hipError_t prepare_sweep2_packed(int storage, size_t lds) {
+    const void *fn = sweep_packed_fn(storage);
+    if (!fn || lds > kLdsLimit) return hipErrorInvalidValue;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+hipError_t packed_grid(int storage, size_t lds, unsigned nblocks, bool persistent, unsigned *grid) {
+    const void *fn = sweep_packed_fn(storage);
+    if (!fn || lds > kLdsLimit) return hipErrorInvalidValue;
+    if (!persistent) {
+        *grid = nblocks;
+        return hipSuccess;
+    }
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kPackedWaves * 64, lds);
+    if (e != hipSuccess) return e;
+    const unsigned resident = (unsigned)std::max(1, cus * std::max(1, per_cu));
+    *grid = std::min(nblocks, resident);
+    return hipSuccess;
+}
+
+hipError_t launch_sweep2_packed(int storage, const SweepJob *jobs, const PackedSub *subs, const PackedBlock *blocks,
+                                unsigned nblocks, unsigned grid, size_t lds, const int64_t *step, int64_t step_now,
+                                hipStream_t s) {
+    if (nblocks == 0) return hipSuccess;
+    const void *fn = sweep_packed_fn(storage);
+    if (!fn || lds > kLdsLimit || grid == 0) return hipErrorInvalidValue;
+    void *args[] = {&jobs, &subs, &blocks, &nblocks, &step, &step_now};
+    hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(kPackedWaves * 64), args, lds, s);
+    return e != hipSuccess ? e : hipGetLastError();
+}
+
+// Streaming copy, 16 bytes per lane, one chunk per thread: the practical HBM
+// ceiling (read + write) that the step kernels are measured against.
+__global__ __launch_bounds__(kBlock) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t s) {
+    if (n16 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)((n16 + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
+    return hipGetLastError();
 }
 
 hipError_t launch_add_step(int64_t *step, int64_t k, hipStream_t s) {

@@ -63,11 +63,28 @@ hipError_t launch_sweep2_batch(int storage, int v, bool big, const SweepJob *job
 // Allow `lds` bytes of dynamic LDS for the batched sweep kernel (call before
 // launching or capturing it).
 hipError_t prepare_sweep2_batch(int storage, int v, bool big, size_t lds);
+// The packed two-step launch (k_sweep2_packed): blocks of
+// packed_block_waves() waves, each sweeping the subs of one PackedBlock side
+// by side; one column per thread, state buffers below 4 GiB.  A sub of a
+// lattice with this storage, ly, class count and rows takes
+// packed_sub_lds(...) bytes of its block's LDS.
+uint32_t packed_block_waves();
+uint32_t packed_sub_lds(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows);
+// Block g of the launch sweeps packed blocks g, g + grid, ...: packed_grid
+// gives one block per packed block, or with `persistent` the blocks the GPU
+// holds at once (CUs x blocks per CU at this LDS size, at most nblocks).
+hipError_t prepare_sweep2_packed(int storage, size_t lds);
+hipError_t packed_grid(int storage, size_t lds, unsigned nblocks, bool persistent, unsigned *grid);
+hipError_t launch_sweep2_packed(int storage, const SweepJob *jobs, const PackedSub *subs, const PackedBlock *blocks,
+                                unsigned nblocks, unsigned grid, size_t lds, const int64_t *step, int64_t step_now,
+                                hipStream_t s);
 // Whether a lattice's state buffers need 64-bit row bases in the sweep
 // kernels (4 GiB or more each).
 bool sweep_big(int storage, const Geom &g);
 // LDS bytes of the sweep kernel for a lattice of this ly, class count and rows per block.
 size_t sweep_lds_bytes(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows);
+// dst[i] = src[i] for n16 16-byte chunks (streaming copy, measurement only).
+hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t s);
 // *step += k (one thread).
 hipError_t launch_add_step(int64_t *step, int64_t k, hipStream_t s);
 

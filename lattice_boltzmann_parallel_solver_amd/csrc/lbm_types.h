@@ -109,4 +109,23 @@ struct SweepJob {
     uint32_t pad = 0;
 };
 
+// Packed sweep launch: each block of kPackedWaves waves sweeps several row
+// ranges (of one or more lattices) side by side, each on its own run of
+// waves and its own part of the block's LDS.
+struct PackedSub {
+    uint32_t job = 0;     // index into the launch's job records
+    uint32_t xa = 0;      // first output row
+    uint32_t rows = 0;    // output rows (clipped at the lattice's end)
+    uint32_t wave0 = 0;   // first wave of the block that sweeps it
+    uint32_t waves = 0;   // waves that sweep it (the row's width in 64-node waves)
+    uint32_t lds_off = 0; // byte offset of its LDS area in the block's
+};
+
+struct PackedBlock {
+    uint32_t sub0 = 0;     // first of its subs in the launch's sub table
+    uint32_t nsub = 0;
+    uint32_t r_block = 0;  // rows of its longest sub: every wave runs r_block + 2 row iterations
+    uint32_t lds = 0;      // LDS bytes its subs use
+};
+
 }  // namespace lbm

@@ -306,7 +306,7 @@ class Batch:
     relaxation of lattices[i].  Close the batch before its lattices."""
 
     def __init__(self, lattices: Sequence[Lattice], omegas: Sequence[float], nodes_per_thread: int = 1,
-                 use_graph: bool = True, temporal: int = 1, rows_per_block: int = 0):
+                 use_graph: bool = True, temporal: int = 1, rows_per_block: int = 0, packed: int = 0):
         if len(lattices) != len(omegas) or not lattices:
             raise ValueError("need one omega per lattice and at least one lattice")
         L = _lib.load()
@@ -318,6 +318,7 @@ class Batch:
         self._h = h
         self.set_launch(nodes_per_thread, use_graph)
         self.set_temporal(temporal, rows_per_block)
+        self.set_packing(packed)
 
     def set_launch(self, nodes_per_thread: int = 1, use_graph: bool = True) -> None:
         check(_lib.load().lbm_batch_set_launch(self._h, int(nodes_per_thread), int(bool(use_graph))))
@@ -326,6 +327,13 @@ class Batch:
         """Two steps per launch for every lattice the sweep kernel supports
         (identical results); the others keep one step per launch."""
         check(_lib.load().lbm_batch_set_temporal(self._h, int(steps_per_launch), int(rows_per_block)))
+
+    def set_packing(self, packed: int = 0) -> None:
+        """With two steps per launch: one launch per group of equal row width
+        (0, the default and the fastest measured), or one packed launch per
+        step pair for the FP32/FP16 lattices of at most 1024 columns (1: one
+        block per packed block, 2: a persistent grid).  Identical results."""
+        check(_lib.load().lbm_batch_set_packing(self._h, int(packed)))
 
     def run(self, steps: int) -> None:
         """Advance every lattice `steps` reference steps (synchronous)."""

@@ -319,3 +319,67 @@ def test_large_batch_groups_on_side_streams_equal_individual_runs():
     check_oracle(lats, want, "f32", probes=True)
     for lat in lats + ref:
         lat.close()
+
+
+@pytest.mark.parametrize("storage,rows", [("f32", 8), ("f32", 3), ("f16", 5)])
+def test_packed_launch_equals_grouped_launches(storage, rows):
+    """One packed launch per step pair (row ranges of many widths side by side
+    in 16-wave blocks; one block per packed block, and a persistent grid)
+    against one launch per group of equal width: states and
+    probe traces equal bit for bit over runs of odd and even length, with a
+    job wider than 1024 columns (two columns per thread, kept in its own
+    group), a job shorter than a row range and a pressure-fix job (one-step
+    kernel) in the same batch; and the FP32 result against the FP64 oracle."""
+    from lattice_boltzmann_parallel_solver_amd import Batch, Lattice
+    from lattice_boltzmann_parallel_solver_amd.boundary_utils import (poiseuille_flow_boundary_conditions,
+                                                                      von_karman_boundary_conditions)
+    from lattice_boltzmann_parallel_solver_amd.initial_values import density_1_velocity_x_u0_velocity_y_0_initial
+
+    shapes = [(120, 51), (97, 300), (2, 64), (64, 514), (33, 129), (5, 1300), (40, 86), (200, 257)]
+
+    def build():
+        lats, oms = [], []
+        for lx, ly in shapes:
+            d = max(2, ly // 4)
+            lat = Lattice(lx, ly, boundary=von_karman_boundary_conditions(lx, ly, 1.0, 0.1, d, walls=True, y_offset=1)
+                          if lx >= 3 else None, storage=storage)
+            rho, u = density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), 0.1)
+            lat.set_equilibrium(rho, u)
+            lat.set_probe(lx // 2, ly // 3, 64)
+            lats.append(lat)
+            oms.append(1.0 / (3 * 0.1 * d / 100.0 + 0.5))
+        lat = Lattice(30, 14, boundary=poiseuille_flow_boundary_conditions(30, 14, 1.0015 / 3, 0.9985 / 3),
+                      storage=storage)
+        lat.set_equilibrium(np.ones((30, 14)), np.zeros((30, 14, 2)))
+        lats.append(lat)
+        oms.append(1.4)
+        return lats, oms
+
+    print(f"case: packed (one block per packed block, persistent) vs grouped batch, {storage}, {rows} rows per range",
+          flush=True)
+    out = []
+    for packed in (1, 0, 2):
+        lats, oms = build()
+        b = Batch(lats, oms, temporal=2, rows_per_block=rows, packed=packed)
+        for chunk in (5, 8, 3, 10):
+            b.run(chunk)
+        b.close()
+        out.append(([lat.state() for lat in lats], [lat.probe() for lat in lats[:-1]]))
+        for lat in lats:
+            lat.close()
+    for (sa, pa), (sb, pb) in zip([out[0], out[2]], [out[1], out[1]]):
+        for k, (a, c) in enumerate(zip(sa, sb)):
+            for x, y in zip(a, c):
+                assert np.array_equal(x, y), k
+        for k, (a, c) in enumerate(zip(pa, pb)):
+            assert np.array_equal(a, c), k
+    if storage == "f32":
+        for k, (lx, ly) in enumerate(shapes):
+            if lx < 3:
+                continue
+            d = max(2, ly // 4)
+            rho, u = O.density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), 0.1)
+            bc = O.von_karman_boundary_conditions(lx, ly, 1.0, 0.1, d, walls=True, y_offset=1)
+            want = N.run(O.equilibrium(rho, u), rho, u, 1.0 / (3 * 0.1 * d / 100.0 + 0.5), 26, bc)
+            for a, w, name in zip(out[0][0][k], want[:3], ("f", "rho", "u")):
+                assert np.abs(a - w).max() <= F32_TOL[name], (k, name)
