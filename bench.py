@@ -14,7 +14,9 @@ roofline   : HBM bound; achieved = algorithmic bytes per launch (every populatio
              FETCH_SIZE/WRITE_SIZE, copy-calibrated
              (profiles/fused_step_8192x2048_hbm_traffic.json).
              single_step_equivalent_gbs: the bandwidth a one-step-per-launch kernel
-             would need for the same MLUPS (72 B per node step).
+             would need for the same MLUPS (72 B per node step); copy_gbs: a
+             1 GiB streaming copy (16 bytes per lane) timed the same way, the
+             practical ceiling, and frac_of_copy = achieved / copy_gbs.
 single_step: the same workload with one step per launch (k_fused).
 fp64, f16  : the same workload with FP64 populations / FP16 deviation storage,
              two steps per launch (FP64 rows of 2048 split into three column
@@ -95,6 +97,40 @@ def build_lattice(storage, torch, temporal=1):
     return lat
 
 
+def copy_gbs(torch, nbytes=1 << 30, reps=10):
+    """HBM bandwidth of the library's streaming copy (16 bytes per lane) over
+    nbytes, read + write counted, timed with events on the current stream:
+    the practical ceiling a streaming kernel is measured against beside the
+    8 TB/s peak.  (torch's copy_ goes through the runtime's blit kernel,
+    which reaches less than the step kernels do.)"""
+    import ctypes
+
+    from lattice_boltzmann_parallel_solver_amd import _lib
+
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    stream = torch.cuda.current_stream().cuda_stream
+    L = _lib.load()
+
+    def once():
+        _lib.check(L.lbm_copy_probe(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), nbytes,
+                                    ctypes.c_void_p(stream)))
+
+    once()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        once()
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end)
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes * reps / (ms / 1e3) / 1e9
+
+
 def time_steps(lat, steps, warmup, torch):
     lat.run(omega(), warmup)
     torch.cuda.synchronize()
@@ -153,7 +189,7 @@ def cpu_model():
     return platform.processor() or platform.machine()
 
 
-def cpu_baseline(steps=4, mp_steps=30, c_steps=120, c1_numpy_steps=400, c1_steps=3000):
+def cpu_baseline(steps=4, mp_steps=30, c_steps=120, c1_numpy_steps=3000, c1_steps=3000):
     """The reference's numpy path restated (the oracle, FP64) on this host's
     cores, each leg sized for a window of at least 10 s at the rates measured
     on the GPU machines' hosts (channel: about 5 MLUPS on one core, 40 MLUPS on
@@ -334,6 +370,7 @@ def main():
     if f16 is not None:
         f16["error_vs_fp64"] = f16_error_note()
 
+    copy = copy_gbs(torch)
     latency = None if args.no_serving else serving_leg()
     batching = None if args.no_batching else batching_leg()
 
@@ -359,7 +396,8 @@ def main():
                      "kernel": "k_sweep2 (two steps per launch)" if spl == 2 else "k_fused (one step per launch)",
                      "bytes_per_node_launch": BYTES_PER_NODE[args.storage],
                      "bytes_per_node_step": BYTES_PER_NODE[args.storage] // spl,
-                     "single_step_equivalent_gbs": mlups * 1e6 * BYTES_PER_NODE[args.storage] / 1e9},
+                     "single_step_equivalent_gbs": mlups * 1e6 * BYTES_PER_NODE[args.storage] / 1e9,
+                     "copy_gbs": copy, "frac_of_copy": achieved / copy},
         "single_step": single,
         "fp64": fp64,
         "fp64_single_step": fp64_single,

@@ -109,15 +109,16 @@ def run_sequential(jobs: Sequence[ChannelJob], steps: int, storage: str = "f32",
 
 def run_batched(jobs: Sequence[ChannelJob], steps: int, storage: str = "f32", nodes_per_thread: Optional[int] = None,
                 use_graph: bool = True, chunk: int = 1000, keep: bool = False, temporal: int = 2,
-                rows_per_block: int = 0):
-    """All channels in one Batch, two steps per launch by default.  Returns
-    (seconds, aggregate MLUPS, lattices or None)."""
+                rows_per_block: int = 0, packed: int = 0):
+    """All channels in one Batch, two steps per launch by default (one launch
+    per group of equal row width; packed=1 one packed launch per step pair,
+    packed=2 persistent).  Returns (seconds, aggregate MLUPS, lattices or None)."""
     lats = [build_lattice(j, storage) for j in jobs]
     oms = [j.omega for j in jobs]
     if nodes_per_thread is None:
         nodes_per_thread = autotune_batch(jobs, storage) if temporal == 1 else 1
     b = Batch(lats, oms, nodes_per_thread=nodes_per_thread, use_graph=use_graph, temporal=temporal,
-              rows_per_block=rows_per_block)
+              rows_per_block=rows_per_block, packed=packed)
     b.run(1)  # first collisions, tables, graph capture
     _sync_all(lats)
     t0 = time.perf_counter()
@@ -238,8 +239,9 @@ def autotune_batch(jobs: Sequence[ChannelJob], storage: str = "f32", sample: int
 def _main(argv=None):
     """Batched channel workload: `python -m lattice_boltzmann_parallel_solver_amd.workloads
     --jobs 256 --steps 20000` prints aggregate MLUPS of the batch (two steps
-    per launch, and one), and of the same channels solved one at a time, each
-    with the autotuned launch shape of its size bucket."""
+    per launch in per-width groups, in one packed launch and in a persistent
+    packed grid, and one step per launch), and of the same channels solved one at a time, each with the
+    autotuned launch shape of its size bucket and with one step per launch."""
     import argparse
     import json
 
@@ -256,8 +258,15 @@ def _main(argv=None):
            "nodes": sum(j.nodes for j in jobs), "nx_range": [min(j.lx for j in jobs), max(j.lx for j in jobs)]}
     print(json.dumps(res), flush=True)
     sec, ml, _ = run_batched(jobs, a.steps, a.storage, temporal=2)
-    res["batched_two_step"] = {"seconds": sec, "mlups": ml}
+    res["batched_two_step"] = {"seconds": sec, "mlups": ml,
+                               "launch": "one launch per group of equal row width, groups on side streams"}
     print(json.dumps({"batched_two_step": res["batched_two_step"]}), flush=True)
+    for mode, name in ((1, "batched_two_step_packed"), (2, "batched_two_step_persistent")):
+        sec, ml, _ = run_batched(jobs, a.steps, a.storage, temporal=2, packed=mode)
+        res[name] = {"seconds": sec, "mlups": ml,
+                     "launch": "one packed launch per step pair, captured in a HIP graph on one stream"
+                               + (", persistent grid" if mode == 2 else "")}
+        print(json.dumps({name: res[name]}), flush=True)
     sec, ml, _ = run_batched(jobs, a.steps, a.storage, temporal=1)
     res["batched_one_step"] = {"seconds": sec, "mlups": ml}
     print(json.dumps({"batched_one_step": res["batched_one_step"]}), flush=True)
@@ -269,6 +278,12 @@ def _main(argv=None):
                                            for k, v in sorted(_LATTICE_CACHE.items())}}
     res["speedup_two_step_vs_sequential"] = res["batched_two_step"]["mlups"] / ml
     res["speedup_one_step_vs_sequential"] = res["batched_one_step"]["mlups"] / ml
+    print(json.dumps({"sequential": {"mlups": ml}}), flush=True)
+    # the plain solve: every lattice alone, one step per launch
+    sec, mlp = run_sequential(seq, a.steps, a.storage, tune=False, temporal=1)
+    res["sequential_plain"] = {"jobs": len(seq), "seconds": sec, "mlups": mlp,
+                               "launch": "one lattice at a time, one step per launch"}
+    res["speedup_two_step_vs_sequential_plain"] = res["batched_two_step"]["mlups"] / mlp
     text = json.dumps(res, indent=1)
     print(text, flush=True)
     if a.out:
