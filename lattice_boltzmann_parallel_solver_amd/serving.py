@@ -469,7 +469,15 @@ def _main(argv=None):
         else:
             pick = [r for kind in ("von_karman", "couette", "poiseuille")
                     for r in [r for r in recs if r.spec.kind == kind][:a.check]]
-        refs = fp64_results([r.spec for r in pick])
+        last[0] = 0.0
+
+        def progress64(t, active, waiting, finished):
+            if t - last[0] >= 10.0:
+                last[0] = t
+                print(json.dumps({"fp64_twins_t_s": round(t, 1), "active": active, "finished": finished}),
+                      flush=True)
+
+        refs = fp64_results([r.spec for r in pick], progress=progress64)
         for r, ref in zip(pick, refs):
             c = compare_to_fp64(r.spec, r.result, ref)
             c["id"] = r.job_id
