@@ -166,8 +166,20 @@ int lbm_set_launch(lbm_ctx *ctx, int cells_per_thread, int xcd_remap, int nt_sto
  * one block (ny > 2048 FP32/FP16, > 1024 FP64) run as column tiles with a
  * halo column each side.  Two steps need no pressure fix in the boundary
  * program, nx >= 3 and at most 128 boundary classes; otherwise LBM_ERR_ARG.
- * No counterpart in the reference (a launch setting of this library). */
+ * steps_per_launch = 3 or 4: that many steps per launch (FP32/FP16, ny <=
+ * 1024, the LDS of K - 1 intermediate stages within 160 KiB: ny <= 1024 for
+ * 3, about 700 for 4); a run's remainder goes two steps and one step at a
+ * time.  No counterpart in the reference (a launch setting of this library). */
 int lbm_set_temporal(lbm_ctx *ctx, int steps_per_launch, int rows_per_block);
+/* Column tiles of the two-step kernel: blocks of at most `threads` threads (a
+ * multiple of 64 up to 1024; 0 = whole rows up to 1024 threads, the default),
+ * each sweeping a tile of columns plus one halo column each side, so that the
+ * row barrier of a small block synchronises only its own waves.
+ * cols_per_thread: 1 or 2 columns per thread in FP32/FP16 tiles (0 = 2; FP64
+ * always 1).  Results are identical for every tile shape.  LBM_ERR_ARG for
+ * other values, or when the tiles would be empty.  No counterpart in the
+ * reference (a launch setting). */
+int lbm_set_sweep_tile(lbm_ctx *ctx, int threads, int cols_per_thread);
 
 /* ---- batches: many lattices advanced by one launch per step ------------- */
 /* A batch groups contexts (same device, same storage) so that one kernel

@@ -95,6 +95,7 @@ _SIGNATURES = {
     "lbm_steps_done": (_I, [_P, ctypes.POINTER(_I64)]),
     "lbm_set_launch": (_I, [_P, _I, _I, _I]),
     "lbm_set_temporal": (_I, [_P, _I, _I]),
+    "lbm_set_sweep_tile": (_I, [_P, _I, _I]),
     "lbm_batch_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double), _I]),
     "lbm_batch_destroy": (_I, [_P]),
     "lbm_batch_set_launch": (_I, [_P, _I, _I]),

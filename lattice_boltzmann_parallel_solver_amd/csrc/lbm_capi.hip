@@ -289,21 +289,48 @@ int lbm_set_launch(lbm_ctx *c, int v, int xcd, int nt) {
 int lbm_set_temporal(lbm_ctx *c, int steps_per_launch, int rows_per_block) {
     int rc = check_ctx(c, "lbm_set_temporal");
     if (rc) return rc;
-    if (steps_per_launch != 1 && steps_per_launch != 2) {
-        set_error("lbm_set_temporal: steps_per_launch must be 1 or 2");
+    if (steps_per_launch < 1 || steps_per_launch > 4) {
+        set_error("lbm_set_temporal: steps_per_launch must be 1, 2, 3 or 4");
         return LBM_ERR_ARG;
     }
     if (rows_per_block < 0) {
         set_error("lbm_set_temporal: rows_per_block must be >= 0 (0 = automatic)");
         return LBM_ERR_ARG;
     }
-    if (steps_per_launch == 2 && !sweep_supported(c->storage, c->g, c->bc.view)) {
+    if (steps_per_launch == 2 && !sweep_supported(c->storage, c->g, c->bc.view, c->cfg.sweep_tile, c->cfg.sweep_v)) {
         set_error("lbm_set_temporal: two steps per launch need no pressure fix, at most 128 boundary classes "
                   "and nx >= 3");
         return LBM_ERR_ARG;
     }
+    if (steps_per_launch >= 3 && !sweepk_supported(c->storage, c->g, c->bc.view, steps_per_launch)) {
+        set_error("lbm_set_temporal: " + std::to_string(steps_per_launch) +
+                  " steps per launch need FP32 or FP16 storage, ny <= 1024, no pressure fix, at most 128 boundary "
+                  "classes, nx >= 3 and their LDS within 160 KiB");
+        return LBM_ERR_ARG;
+    }
     c->cfg.steps_per_launch = steps_per_launch;
     c->cfg.sweep_rows = rows_per_block;
+    return LBM_OK;
+}
+
+int lbm_set_sweep_tile(lbm_ctx *c, int threads, int cols_per_thread) {
+    int rc = check_ctx(c, "lbm_set_sweep_tile");
+    if (rc) return rc;
+    if (threads < 0 || threads % 64 != 0 || threads > kSweepThreadsMax) {
+        set_error("lbm_set_sweep_tile: threads must be 0 (whole rows) or a multiple of 64 up to 1024");
+        return LBM_ERR_ARG;
+    }
+    if (cols_per_thread < 0 || cols_per_thread > 2) {
+        set_error("lbm_set_sweep_tile: cols_per_thread must be 0 (automatic), 1 or 2");
+        return LBM_ERR_ARG;
+    }
+    if (c->cfg.steps_per_launch == 2 && !sweep_supported(c->storage, c->g, c->bc.view, threads, cols_per_thread)) {
+        set_error("lbm_set_sweep_tile: no column tiles of " + std::to_string(threads) + " threads for ny = " +
+                  std::to_string(c->ny));
+        return LBM_ERR_ARG;
+    }
+    c->cfg.sweep_tile = threads;
+    c->cfg.sweep_v = cols_per_thread;
     return LBM_OK;
 }
 
@@ -401,7 +428,19 @@ int lbm_run(lbm_ctx *c, double omega, int64_t nsteps) {
         left--;
     }
     const bool probing = c->probe.cell >= 0;
-    if (c->cfg.steps_per_launch == 2 && sweep_supported(c->storage, c->g, c->bc.view)) {
+    const int K = c->cfg.steps_per_launch;
+    if (K >= 3 && sweepk_supported(c->storage, c->g, c->bc.view, K)) {
+        for (; left >= K; left -= K) {
+            // materialises f_{steps} .. f_{steps+K-1}
+            const int64_t slot = probing ? c->steps - c->probe_base - 1 : -1;
+            LBM_HIP_CHECK(launch_sweepk(c->storage, K, c->g, c->st, c->cur, c->bc.view, omega, c->probe, slot,
+                                        c->cfg, c->stream),
+                          "K-step sweep kernel");
+            c->cur ^= 1;
+            c->steps += K;
+        }
+    }
+    if (K >= 2 && sweep_supported(c->storage, c->g, c->bc.view, c->cfg.sweep_tile, c->cfg.sweep_v)) {
         for (; left >= 2; left -= 2) {
             // materialises f_{steps} and f_{steps+1}
             const int64_t slot = probing ? c->steps - c->probe_base - 1 : -1;

@@ -775,7 +775,7 @@ __global__ void k_next_step(int64_t *step) { *step += 1; }
 // row's barrier).  Not used with the pressure fix,
 // whose producers write rows owned by other blocks.
 // ---------------------------------------------------------------------------
-constexpr int kSweepThreads = 1024;
+constexpr int kSweepThreads = kSweepThreadsMax;
 // y-moving directions: the first three arrive from y - 1, the last three from y + 1.
 constexpr int kYM[6] = {2, 5, 6, 4, 7, 8};
 constexpr int kYSlot[9] = {-1, -1, 0, -1, 3, 1, 2, 4, 5};
@@ -1358,6 +1358,375 @@ __global__ __launch_bounds__(kPackedWaves * 64, 8) void k_sweep2_packed(const Sw
     }
 }
 
+// ---------------------------------------------------------------------------
+// K steps per launch (K >= 3), one column per thread over whole rows
+// (ly <= 1024), FP32/FP16 storage.  The two-step sweep's pipeline with K
+// stages: at row iteration it, stage 1 collides row x = xa - (K-1) + it and
+// publishes its y-moving populations in LDS; after the one barrier of the
+// iteration, stage s >= 2 collides row x - (2s - 3) from the y-streamed output
+// of stage s - 1 (stage 2 reads what stage 1 published before this barrier,
+// stage s >= 3 what stage s - 1 published after the previous one), the last
+// stage stores to HBM and the others publish in LDS for the next stage.
+// Stages run from the last to the second, so that a stage reads its own node's
+// previous-stage values (its reflection sources) before that stage overwrites
+// them.  Stage s covers rows [xa - (K - s), xa + R + K - s), so the input is
+// read once plus K rows on each side, the output written once per K steps:
+// 36 / K B per node and step in FP32.  Every value goes through the same
+// arithmetic as in single steps (identical results).  Needs R >= K - 1 rows in
+// every block; no pressure fix, at most kSweepMaxClasses classes.
+// ---------------------------------------------------------------------------
+constexpr int kSweepMaxK = 4;
+
+template <class S>
+struct SweepKParams {
+    const S *src;
+    S *dst;
+    const typename Io<S>::C *side_prev;
+    typename Io<S>::C *side_next;
+    Geom g;
+    BcView bc;
+    double omega;
+    int64_t probe_cell;   // -1: no probe
+    double *probe_ring;   // samples of step s (1-based) go to slot probe_slot + s - 1
+    int64_t probe_cap;
+    int64_t probe_slot;
+    uint32_t rows;        // output rows per block
+    S *scratch;           // 9 blockDim.x values, written and never read
+};
+
+// Dynamic LDS of the K-step kernel: per stage s < K two exchange buffers of six
+// planes of ly + 1 values and the stage's outlet values (3 per column), then
+// the incoming outlet values, the class table and one "special" byte per
+// (stage-1 row, wave).
+struct SweepKLds {
+    size_t ex_stride, side, side_in, cls, flag, total;
+};
+
+template <class S>
+__host__ __device__ inline SweepKLds sweepk_lds_layout(uint32_t ly, uint32_t nclasses, uint32_t rows, int K) {
+    using C = typename Io<S>::C;
+    SweepKLds L;
+    L.ex_stride = align16(12 * ((size_t)ly + 1) * sizeof(S));
+    L.side = L.ex_stride * (K - 1);
+    L.side_in = align16(L.side + (size_t)(K - 1) * 3 * ly * sizeof(C));
+    L.cls = align16(L.side_in + 3 * (size_t)ly * sizeof(C));
+    L.flag = align16(L.cls + (size_t)nclasses * sizeof(ClassEntry));
+    L.total = L.flag + (size_t)(rows + 2 * (K - 1)) * ((ly + 63) / 64);
+    return L;
+}
+
+template <int B>
+struct IntC {
+    static constexpr int value = B;
+};
+
+template <class S, int K, bool kBig>
+__device__ __forceinline__ void sweepk_block(const SweepKParams<S> &p, uint32_t xa, unsigned char *smem) {
+    using C = typename Io<S>::C;
+    static_assert(K >= 3 && K <= kSweepMaxK, "K-step sweep: 3 <= K <= kSweepMaxK");
+    const Geom &g = p.g;
+    const uint32_t ly = g.ly, lx = g.lx, pitch = g.pitch;
+    const uint32_t T = blockDim.x, tid = threadIdx.x;
+    using GS = LBM_GLOBAL S;
+    const GS *gsrc = (const GS *)p.src;
+    GS *gdst = (GS *)p.dst;
+    GS *gscr = (GS *)p.scratch;
+    LBM_GLOBAL C *gside = (LBM_GLOBAL C *)p.side_next;
+    const LBM_GLOBAL uint16_t *gcodes = (const LBM_GLOBAL uint16_t *)p.bc.codes;
+    const uint32_t R = min(p.rows, lx - xa);
+    const uint32_t nw = (ly + 63) / 64;
+    const uint32_t cs = ly + 1;  // LDS plane stride (a spare column)
+    const uint32_t nrows1 = R + 2 * (K - 1);  // stage-1 rows
+    const bool has_bc = p.bc.chunk != nullptr;
+    const SweepKLds L = sweepk_lds_layout<S>(ly, has_bc ? p.bc.nclasses : 0, p.rows, K);
+    C *side_in = reinterpret_cast<C *>(smem + L.side_in);
+    ClassEntry *cls = reinterpret_cast<ClassEntry *>(smem + L.cls);
+    uint8_t *wflag = smem + L.flag;
+    // exchange buffer b of stage s (1 <= s < K), and the stage's outlet values
+    auto exb = [&](int s, uint32_t b) { return reinterpret_cast<S *>(smem + (size_t)(s - 1) * L.ex_stride) + b * 6 * (size_t)cs; };
+    auto sideb = [&](int s) { return reinterpret_cast<C *>(smem + L.side) + (size_t)(s - 1) * 3 * ly; };
+    // stage-1 row of iteration it (periodic)
+    auto row_at = [&](uint32_t it) {
+        const uint32_t r = xa + it + lx - (K - 1);
+        return r >= lx ? (r - lx >= lx ? r - 2 * lx : r - lx) : r;
+    };
+    if (has_bc) {
+        const uint32_t words = p.bc.nclasses * (uint32_t)(sizeof(ClassEntry) / 4);
+        for (uint32_t i = tid; i < words; i += T)
+            reinterpret_cast<uint32_t *>(cls)[i] = ((const LBM_GLOBAL uint32_t *)p.bc.classes)[i];
+        // stage 1 reads the previous launch's outlet values from LDS: blocks
+        // whose stage-1 rows include lx - 1 need them
+        if ((p.bc.flags & kProgOutlet) && (xa < (uint32_t)(K - 1) || xa + R + K - 1 >= lx)) {
+            for (uint32_t i = tid; i < 3 * ly; i += T) side_in[i] = ((const LBM_GLOBAL C *)p.side_prev)[i];
+        }
+        for (uint32_t i = tid; i < nrows1 * nw; i += T) {
+            const uint32_t rr = i / nw, j = i - rr * nw;
+            wflag[i] = ((const LBM_GLOBAL uint8_t *)p.bc.wave)[(size_t)row_at(rr) * g.wpr + j];
+        }
+    }
+    __syncthreads();
+    const uint32_t y = min(tid, ly - 1);
+    const uint32_t ym = y == 0 ? ly - 1 : y - 1, yp = y + 1 >= ly ? 0 : y + 1;
+    const uint32_t jw = min(tid >> 6, nw - 1);
+    const uint32_t col = min(tid, ly);  // LDS column written: lanes past the last column use the spare one
+    auto special = [&](uint32_t rr) -> bool {
+        if (!has_bc) return false;
+        return __builtin_amdgcn_readfirstlane(wflag[rr * nw + jw]) != 0;
+    };
+    auto load_code = [&](uint32_t rr) -> uint32_t {
+        if (!special(rr)) return 0;
+        return gcodes[row_at(rr) * ly + y];
+    };
+    auto pull_row = [&](uint32_t rr, uint32_t code, C out[9]) {
+        const uint32_t xr = row_at(rr), xm = xr == 0 ? lx - 1 : xr - 1, xp = xr + 1 == lx ? 0 : xr + 1;
+        if (special(rr)) {
+            const uint32_t w = code ? cls[code].src : 0u;
+            pull32_bc<S, kBig>(gsrc, pitch, xm, xr, xp, y, ym, yp, w, out);
+        } else {
+            pull32<S, kBig>(gsrc, pitch, xm, xr, xp, y, ym, yp, out);
+        }
+    };
+    auto probe_at = [&](int s) -> LBM_GLOBAL double * {
+        const int64_t slot = p.probe_slot + s - 1;
+        return slot >= 0 ? (LBM_GLOBAL double *)p.probe_ring + 3 * (slot % p.probe_cap) : nullptr;
+    };
+    // codes: c0 = stage-1 row of this iteration, c1 = the prefetched row, c2 =
+    // the row after it; hist[j] = code of the stage-1 row of iteration it-1-j
+    uint32_t c0 = load_code(0), c1 = load_code(1), c2 = 0;
+    uint32_t hist[2 * K - 3];
+#pragma unroll
+    for (int j = 0; j < 2 * K - 3; j++) hist[j] = 0;
+    // per stage s >= 2 (index s - 2): y-streamed output of stage s - 1 for
+    // rows r - 1 (A: directions 1, 5, 8) and r (B: 0, 2, 4; Cc: 1, 5, 8; D: 3)
+    C A[K - 1][3], B[K - 1][3], Cc[K - 1][3], D[K - 1];
+    // directions 0, 1, 3 of the output of stage s (s < K) published in the
+    // previous iteration (index s - 1); stage s + 1 reads them with its LDS row
+    C P013[K - 1][3];
+#pragma unroll
+    for (int s = 0; s < K - 1; s++) {
+        D[s] = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) A[s][j] = B[s][j] = Cc[s][j] = P013[s][j] = 0;
+    }
+    C pf[9];
+    pull_row(0, c0, pf);
+
+    // one iteration; kPf: prefetch the next stage-1 row; kS1: stage 1 runs;
+    // stage s >= 2 runs where bit s of kMask is set
+    auto body = [&](uint32_t it, auto pf_flag, auto s1_flag, auto mask_tag) {
+        constexpr bool kPf = decltype(pf_flag)::value;
+        constexpr bool kS1 = decltype(s1_flag)::value;
+        constexpr int kMask = decltype(mask_tag)::value;
+        C v[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) v[i] = pf[i];
+        if constexpr (kPf) {
+            pull_row(it + 1, c1, pf);
+            c2 = it + 2 < nrows1 ? load_code(it + 2) : 0u;
+        }
+        C s013[3] = {0, 0, 0};
+        if constexpr (kS1) {
+            const uint32_t x = row_at(it);
+            uint32_t flags = 0;
+            if (c0) {
+                const ClassEntry &ce = cls[c0];
+                const uint32_t w = ce.src;
+                flags = ce.flags;
+#pragma unroll
+                for (int d = 0; d < 9; d++) {
+                    const uint32_t kind = (w >> (2 * d)) & 3u;
+                    if (kind == kSrcReflect) {
+                        v[d] = v[d] - (C)ce.k[d];  // v holds the node's own opposite population
+                    } else if (kind == kSrcConst) {
+                        if constexpr (Io<S>::kDev) v[d] = (C)(ce.k[d] - kWd[d]);
+                        else v[d] = ce.k[d];
+                    } else if (kind == kSrcOutlet) {
+                        v[d] = side_in[(size_t)y * 3 + outlet_slot(d)];
+                    }
+                }
+            }
+            NodeOut<S> o;
+            collide_node<S>(v, p.omega, o);
+            if (flags & kFlagOutletSide) {
+                C *sd = sideb(1);
+                sd[(size_t)y * 3 + 0] = v[3];
+                sd[(size_t)y * 3 + 1] = v[6];
+                sd[(size_t)y * 3 + 2] = v[7];
+            }
+            if (p.probe_cell >= 0 && it >= K - 1 && it < R + K - 1 && tid < ly &&
+                (int64_t)(x * ly + y) == p.probe_cell) {
+                if (LBM_GLOBAL double *pr = probe_at(1)) {
+                    pr[0] = o.rho;
+                    pr[1] = (double)o.ux;
+                    pr[2] = (double)o.uy;
+                }
+            }
+            S *e1 = exb(1, it & 1);
+#pragma unroll
+            for (int j = 0; j < 6; j++) lds_put<S>(e1 + j * cs + col, o.s[kYM[j]]);
+            s013[0] = round_to_storage<S>(o.s[0]);
+            s013[1] = round_to_storage<S>(o.s[1]);
+            s013[2] = round_to_storage<S>(o.s[3]);
+        }
+        __syncthreads();
+        // y-streamed input of every stage s >= 2, read before any stage publishes
+        C Y[K - 1][9];
+#pragma unroll
+        for (int s = 2; s <= K; s++) {
+            const S *e = s == 2 ? exb(1, it & 1) : exb(s - 1, (it + 1) & 1);
+            C *Ys = Y[s - 2];
+            if (s == 2) {
+                Ys[0] = s013[0];
+                Ys[1] = s013[1];
+                Ys[3] = s013[2];
+            } else {
+                Ys[0] = P013[s - 2][0];
+                Ys[1] = P013[s - 2][1];
+                Ys[3] = P013[s - 2][2];
+            }
+            Ys[2] = lds_get<S>(e + 0 * cs + ym);
+            Ys[5] = lds_get<S>(e + 1 * cs + ym);
+            Ys[6] = lds_get<S>(e + 2 * cs + ym);
+            Ys[4] = lds_get<S>(e + 3 * cs + yp);
+            Ys[7] = lds_get<S>(e + 4 * cs + yp);
+            Ys[8] = lds_get<S>(e + 5 * cs + yp);
+        }
+#pragma unroll
+        for (int s = K; s >= 2; s--) {
+            const int q = s - 2;
+            if ((kMask >> s) & 1) {
+                const uint32_t r = row_at(it - (2 * s - 3));
+                const uint32_t code = hist[2 * s - 4];
+                C w2[9];
+                w2[0] = B[q][0];
+                w2[2] = B[q][1];
+                w2[4] = B[q][2];
+                w2[1] = A[q][0];
+                w2[5] = A[q][1];
+                w2[8] = A[q][2];
+                w2[3] = Y[q][3];
+                w2[6] = Y[q][6];
+                w2[7] = Y[q][7];
+                uint32_t flags2 = 0;
+                if (code) {
+                    // reflections read the node's own stage s-1 output: directions
+                    // 0, 1, 3 from registers, the y-moving ones from the LDS row
+                    // stage s-1 published for this row (stage 1: one iteration
+                    // ago; stage s-1 >= 2: two iterations ago, same buffer parity)
+                    const S *own = s == 2 ? exb(1, (it + 1) & 1) : exb(s - 1, it & 1);
+                    const ClassEntry &ce = cls[code];
+                    const uint32_t w = ce.src;
+                    flags2 = ce.flags;
+#pragma unroll
+                    for (int d = 0; d < 9; d++) {
+                        const uint32_t kind = (w >> (2 * d)) & 3u;
+                        if (kind == kSrcReflect) {
+                            const int so = kOpp[d];
+                            const C ov = so == 0 ? B[q][0] : so == 1 ? Cc[q][0] : so == 3 ? D[q]
+                                                                                    : lds_get<S>(own + kYSlot[so] * cs + y);
+                            w2[d] = ov - (C)ce.k[d];
+                        } else if (kind == kSrcConst) {
+                            if constexpr (Io<S>::kDev) w2[d] = (C)(ce.k[d] - kWd[d]);
+                            else w2[d] = ce.k[d];
+                        } else if (kind == kSrcOutlet) {
+                            w2[d] = sideb(s - 1)[(size_t)y * 3 + outlet_slot(d)];
+                        }
+                    }
+                }
+                NodeOut<S> o2;
+                collide_node<S>(w2, p.omega, o2);
+                const bool own_row = s == K || (it >= (uint32_t)(2 * s - 3 + K - 1) && it < R + 2 * s - 3 + K - 1);
+                if (p.probe_cell >= 0 && own_row && tid < ly && (int64_t)(r * ly + y) == p.probe_cell) {
+                    if (LBM_GLOBAL double *pr = probe_at(s)) {
+                        pr[0] = o2.rho;
+                        pr[1] = (double)o2.ux;
+                        pr[2] = (double)o2.uy;
+                    }
+                }
+                if (s == K) {
+                    // every direction is stored; lanes past column ly-1 store its
+                    // values into the row padding (never read)
+                    const uint32_t yst = min(tid, pitch - 1);
+#pragma unroll
+                    for (int d = 0; d < 9; d++) Io<S>::st(pop32<kBig>(gdst, pitch, r, d, yst), o2.s[d], false);
+                    if ((flags2 & kFlagOutletSide) && tid < ly) {
+                        gside[(size_t)y * 3 + 0] = w2[3];
+                        gside[(size_t)y * 3 + 1] = w2[6];
+                        gside[(size_t)y * 3 + 2] = w2[7];
+                    }
+                } else {
+                    if (flags2 & kFlagOutletSide) {
+                        C *sd = sideb(s);
+                        sd[(size_t)y * 3 + 0] = w2[3];
+                        sd[(size_t)y * 3 + 1] = w2[6];
+                        sd[(size_t)y * 3 + 2] = w2[7];
+                    }
+                    S *es = exb(s, it & 1);
+#pragma unroll
+                    for (int j = 0; j < 6; j++) lds_put<S>(es + j * cs + col, o2.s[kYM[j]]);
+                    P013[s - 1][0] = round_to_storage<S>(o2.s[0]);
+                    P013[s - 1][1] = round_to_storage<S>(o2.s[1]);
+                    P013[s - 1][2] = round_to_storage<S>(o2.s[3]);
+                }
+            }
+        }
+        // shift the carried rows of every stage
+#pragma unroll
+        for (int q = 0; q < K - 1; q++) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) A[q][j] = Cc[q][j];
+            B[q][0] = Y[q][0];
+            B[q][1] = Y[q][2];
+            B[q][2] = Y[q][4];
+            Cc[q][0] = Y[q][1];
+            Cc[q][1] = Y[q][5];
+            Cc[q][2] = Y[q][8];
+            D[q] = Y[q][3];
+        }
+#pragma unroll
+        for (int j = 2 * K - 4; j > 0; j--) hist[j] = hist[j - 1];
+        hist[0] = c0;
+        c0 = c1;
+        c1 = c2;
+    };
+    // prologue: iterations 0 .. 3K-5 (stage s >= 2 from iteration 3s - 4 on),
+    // prefetching; R >= K - 1 keeps all of them before the epilogue
+    auto prologue = [&](auto it_tag) {
+        constexpr int it = decltype(it_tag)::value;
+        constexpr int mask = (it >= 2 ? 4 : 0) | (K >= 3 && it >= 5 ? 8 : 0) | (K >= 4 && it >= 8 ? 16 : 0);
+        body((uint32_t)it, Flag<true>{}, Flag<true>{}, IntC<mask>{});
+    };
+    prologue(IntC<0>{});
+    prologue(IntC<1>{});
+    prologue(IntC<2>{});
+    prologue(IntC<3>{});
+    prologue(IntC<4>{});
+    if constexpr (K >= 4) {
+        prologue(IntC<5>{});
+        prologue(IntC<6>{});
+        prologue(IntC<7>{});
+    }
+    // as many stores on this path as on the loop's back edge (see sweep_block)
+#pragma unroll
+    for (int q = 0; q < 9; q++) Io<S>::st(at32(gscr, q * T + tid), (C)0, false);
+    constexpr int kAll = (1 << (K + 1)) - 4;  // bits 2 .. K
+    for (uint32_t it = 3 * K - 4; it + 3 <= R + 2 * K - 1; it++) body(it, Flag<true>{}, Flag<true>{}, IntC<kAll>{});
+    // epilogue: iteration R + 2K - 3 + e; stage 1 only at e = 0, stage s while e <= s - 2
+    const uint32_t e0 = R + 2 * K - 3;
+    body(e0, Flag<false>{}, Flag<true>{}, IntC<kAll>{});
+    body(e0 + 1, Flag<false>{}, Flag<false>{}, IntC<kAll & ~4>{});
+    if constexpr (K >= 4) body(e0 + 2, Flag<false>{}, Flag<false>{}, IntC<kAll & ~12>{});
+}
+
+// Three steps with 32-bit offsets fit 96 registers without spilling: five
+// waves per SIMD, so that two blocks of up to ten waves share a CU.
+template <class S, int K, bool kBig>
+__global__ __launch_bounds__(kSweepThreads, K == 3 && !kBig ? 5 : 4) void k_sweepk(SweepKParams<S> p) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    sweepk_block<S, K, kBig>(p, blk * p.rows, smem);
+}
+
 __global__ void k_add_step(int64_t *step, int64_t k) { *step += k; }
 
 // ---------------------------------------------------------------------------
@@ -1770,7 +2139,7 @@ hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView
     p.xcd = cfg.xcd_remap;
     p.scratch = static_cast<S *>(st.scratch);
     const SweepShape sh = sweep_shape(sizeof(S) == 8 ? LBM_STORE_F64 : (sizeof(S) == 4 ? LBM_STORE_F32 : LBM_STORE_F16),
-                                      g.ly);
+                                      g.ly, cfg.sweep_tile, cfg.sweep_v);
     const int V = sh.v;
     const unsigned threads = sh.threads;
     const bool tiled = sh.ntiles > 1;
@@ -1809,11 +2178,15 @@ hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView
 
 }  // namespace
 
-SweepShape sweep_shape(int storage, uint32_t ly) {
+SweepShape sweep_shape(int storage, uint32_t ly, int tile_threads, int tile_v) {
     // one column per thread for FP64, two for FP32/FP16: the shapes that fit
-    // the 128 registers of a 1024-thread block without spilling
-    const uint32_t vmax = storage == LBM_STORE_F64 ? 1 : 2;
-    const uint32_t maxc = (uint32_t)kSweepThreads * vmax;
+    // the 128 registers of a 1024-thread block without spilling.  A block
+    // holds at most tmax threads: 1024, or the tile width asked for; FP32/FP16
+    // tiles may ask for one column per thread
+    const uint32_t vmax = storage == LBM_STORE_F64 || tile_v == 1 ? 1 : 2;
+    const uint32_t tmax = tile_threads > 0 ? std::min((uint32_t)tile_threads, (uint32_t)kSweepThreads)
+                                           : (uint32_t)kSweepThreads;
+    const uint32_t maxc = tmax * vmax;
     SweepShape sh;
     if (ly <= maxc) {
         sh.ntiles = 1;
@@ -1824,16 +2197,18 @@ SweepShape sweep_shape(int storage, uint32_t ly) {
         sh.tw = (ly + sh.ntiles - 1) / sh.ntiles;
         sh.lw = sh.tw + 2;
     }
-    sh.v = (storage == LBM_STORE_F64 || sh.lw <= (uint32_t)kSweepThreads) ? 1 : 2;
+    sh.v = (storage == LBM_STORE_F64 || sh.lw <= tmax) ? 1 : 2;
     sh.threads = ((sh.lw + sh.v - 1) / sh.v + 63) / 64 * 64;
     return sh;
 }
 
-bool sweep_supported(int storage, const Geom &g, const BcView &bc) {
+bool sweep_supported(int storage, const Geom &g, const BcView &bc, int tile_threads, int tile_v) {
     if (bc.flags & kProgPressure) return false;
     if (bc.chunk && bc.nclasses > kSweepMaxClasses) return false;
     if (g.lx < 3 || g.ly < 1) return false;
-    const SweepShape sh = sweep_shape(storage, g.ly);
+    if (tile_threads < 0 || tile_threads % 64 != 0 || tile_threads > kSweepThreads) return false;
+    if (tile_v < 0 || tile_v > 2) return false;
+    const SweepShape sh = sweep_shape(storage, g.ly, tile_threads, tile_v);
     // every tile holds at least one output column
     if (sh.ntiles > 1 && (uint64_t)sh.tw * (sh.ntiles - 1) >= g.ly) return false;
     return sweep_lds_for(storage, sh.lw, bc.chunk ? bc.nclasses : 0, kSweepMaxRows) <= kLdsLimit;
@@ -1866,13 +2241,104 @@ int sweep_rows_for(const Geom &g, const LaunchCfg &cfg, int slots) {
 hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int src, const BcView &bc,
                          double omega, const ProbeView &pr, int64_t slot1, int64_t slot2, const LaunchCfg &cfg,
                          hipStream_t s) {
-    if (!sweep_supported(storage, g, bc)) return hipErrorInvalidValue;
+    if (!sweep_supported(storage, g, bc, cfg.sweep_tile, cfg.sweep_v)) return hipErrorInvalidValue;
     switch (storage) {
         case LBM_STORE_F64: return sweep_typed<double>(g, st, src, bc, omega, pr, slot1, slot2, cfg, s);
         case LBM_STORE_F32: return sweep_typed<float>(g, st, src, bc, omega, pr, slot1, slot2, cfg, s);
         case LBM_STORE_F16: return sweep_typed<__half>(g, st, src, bc, omega, pr, slot1, slot2, cfg, s);
     }
     return hipErrorInvalidValue;
+}
+
+namespace {
+template <class S>
+const void *sweepk_fn(int K, bool big) {
+    switch (K) {
+        case 3: return big ? reinterpret_cast<const void *>(k_sweepk<S, 3, true>)
+                           : reinterpret_cast<const void *>(k_sweepk<S, 3, false>);
+        case 4: return big ? reinterpret_cast<const void *>(k_sweepk<S, 4, true>)
+                           : reinterpret_cast<const void *>(k_sweepk<S, 4, false>);
+    }
+    return nullptr;
+}
+
+size_t sweepk_lds_for(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows, int K) {
+    return storage == LBM_STORE_F32 ? sweepk_lds_layout<float>(ly, nclasses, rows, K).total
+                                    : sweepk_lds_layout<__half>(ly, nclasses, rows, K).total;
+}
+
+template <class S>
+hipError_t sweepk_typed(int K, const Geom &g, const StateView &st, int src, const BcView &bc, double omega,
+                        const ProbeView &pr, int64_t slot, const LaunchCfg &cfg, hipStream_t s) {
+    using C = typename Io<S>::C;
+    constexpr int storage = sizeof(S) == 4 ? LBM_STORE_F32 : LBM_STORE_F16;
+    SweepKParams<S> p;
+    p.src = static_cast<const S *>(st.plane[src]);
+    p.dst = static_cast<S *>(st.plane[src ^ 1]);
+    p.side_prev = static_cast<const C *>(st.side[src]);
+    p.side_next = static_cast<C *>(st.side[src ^ 1]);
+    p.g = g;
+    p.bc = bc;
+    p.omega = omega;
+    const bool probing = pr.cell >= 0 && pr.ring && pr.capacity > 0 && slot + K - 1 >= 0;
+    p.probe_cell = probing ? pr.cell : -1;
+    p.probe_ring = pr.ring;
+    p.probe_cap = pr.capacity;
+    p.probe_slot = slot;
+    p.scratch = static_cast<S *>(st.scratch);
+    const unsigned threads = (g.ly + 63) / 64 * 64;
+    if (!p.scratch || (size_t)9 * threads * sizeof(S) > kSweepScratchBytes) return hipErrorInvalidValue;
+    const void *fn = sweepk_fn<S>(K, sweep_big(storage, g));
+    if (!fn) return hipErrorInvalidValue;
+    const uint32_t ncls = bc.chunk ? bc.nclasses : 0;
+    int slots = 0;
+    if (cfg.sweep_rows <= 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        const size_t lds_est = sweepk_lds_layout<S>(g.ly, ncls, 64, K).total;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_est) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds_est) == hipSuccess)
+            slots = std::max(1, cus * (per_cu > 0 ? per_cu : 1));
+    }
+    p.rows = (uint32_t)sweepk_rows(g, sweep_rows_for(g, cfg, slots), K);
+    const size_t lds = sweepk_lds_layout<S>(g.ly, ncls, p.rows, K).total;
+    if (lds > kLdsLimit) return hipErrorInvalidValue;
+    const unsigned grid = (g.lx + p.rows - 1) / p.rows;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    void *args[] = {&p};
+    e = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, s);
+    return e != hipSuccess ? e : hipGetLastError();
+}
+}  // namespace
+
+bool sweepk_supported(int storage, const Geom &g, const BcView &bc, int K) {
+    if (K < 3 || K > kSweepMaxK) return false;
+    if (storage != LBM_STORE_F32 && storage != LBM_STORE_F16) return false;
+    if (bc.flags & kProgPressure) return false;
+    if (bc.chunk && bc.nclasses > kSweepMaxClasses) return false;
+    if (g.lx < 3 || g.ly < 1 || g.ly > (uint32_t)kSweepThreads) return false;
+    return sweepk_lds_for(storage, g.ly, bc.chunk ? bc.nclasses : 0, kSweepMaxRows + 4, K) <= kLdsLimit;
+}
+
+int sweepk_rows(const Geom &g, int rows, int K) {
+    const int lx = (int)g.lx;
+    int r = std::max(rows, K - 1);
+    while (r < lx) {
+        const int last = lx - (lx + r - 1) / r * r + r;  // rows of the last block
+        if (last >= K - 1) break;
+        r++;
+    }
+    return std::min(r, lx);
+}
+
+hipError_t launch_sweepk(int storage, int K, const Geom &g, const StateView &st, int src, const BcView &bc,
+                         double omega, const ProbeView &pr, int64_t probe_slot, const LaunchCfg &cfg,
+                         hipStream_t s) {
+    if (!sweepk_supported(storage, g, bc, K)) return hipErrorInvalidValue;
+    if (storage == LBM_STORE_F32) return sweepk_typed<float>(K, g, st, src, bc, omega, pr, probe_slot, cfg, s);
+    return sweepk_typed<__half>(K, g, st, src, bc, omega, pr, probe_slot, cfg, s);
 }
 
 namespace {

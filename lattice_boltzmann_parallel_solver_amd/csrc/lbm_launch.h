@@ -33,10 +33,23 @@ hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int sr
 // offsets, and its LDS (12 (cols + 1) storage values, 6 cols compute values,
 // the class table, one byte per row and wave column; cols = ly or a column
 // tile's width plus 2) within 160 KiB.
-bool sweep_supported(int storage, const Geom &g, const BcView &bc);
+bool sweep_supported(int storage, const Geom &g, const BcView &bc, int tile_threads = 0, int tile_v = 0);
+// K (3 or 4) fused steps in one launch: the row sweep with K stages, one
+// column per thread over whole rows (ly <= 1024), FP32/FP16 storage.  The
+// samples of step s (1-based) go to ring slot probe_slot + s - 1 (< 0: none).
+// hipErrorInvalidValue if sweepk_supported() is false.
+hipError_t launch_sweepk(int storage, int K, const Geom &g, const StateView &st, int src, const BcView &bc,
+                         double omega, const ProbeView &pr, int64_t probe_slot, const LaunchCfg &cfg,
+                         hipStream_t s);
+// As sweep_supported(), and ly <= 1024, FP32 or FP16, its LDS (K - 1 stages of
+// exchange and outlet buffers) within 160 KiB.
+bool sweepk_supported(int storage, const Geom &g, const BcView &bc, int K);
+// Rows per block of the K-step kernel: `rows`, raised until every block
+// (the last one included) holds at least K - 1 rows.
+int sweepk_rows(const Geom &g, int rows, int K);
 // The batched sweep takes whole rows only (no column tiles).
 bool sweep_batch_supported(int storage, const Geom &g, const BcView &bc);
-SweepShape sweep_shape(int storage, uint32_t ly);
+SweepShape sweep_shape(int storage, uint32_t ly, int tile_threads = 0, int tile_v = 0);
 // Output rows per block of the sweep kernel: cfg.sweep_rows if set, else
 // spread over whole waves of `slots` resident blocks (at least 8 rows).
 int sweep_rows_for(const Geom &g, const LaunchCfg &cfg, int slots);

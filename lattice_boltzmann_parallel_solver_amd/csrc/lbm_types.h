@@ -48,6 +48,8 @@ struct StateView {
 // Write-only scratch of a context: the sweep kernel's entry-path stores land
 // here (18 values per thread of a 1024-thread block at most).
 constexpr size_t kSweepScratchBytes = 18 * 1024 * 8;
+// Most threads of one block of the two-step sweep kernel.
+constexpr int kSweepThreadsMax = 1024;
 
 struct ProbeView {
     int64_t cell = -1;       // node index, -1 = off
@@ -61,6 +63,8 @@ struct LaunchCfg {
     int nt_stores = 0;
     int steps_per_launch = 1;  // 2: two-step row-sweep kernel where supported
     int sweep_rows = 0;        // output rows per block of the sweep kernel (0: automatic)
+    int sweep_tile = 0;        // threads per column tile of the sweep kernel (0: whole rows up to one block)
+    int sweep_v = 0;           // columns per thread of column tiles (0: two for FP32/FP16, one for FP64)
 };
 
 // Launch shape of the sweep kernel for rows of ly nodes: columns per thread,

@@ -163,6 +163,13 @@ class Lattice:
         (pressure fix, fewer than three rows, more than 128 boundary classes)."""
         check(_lib.load().lbm_set_temporal(self._h, int(steps_per_launch), int(rows_per_block)))
 
+    def set_sweep_tile(self, threads: int = 0, cols_per_thread: int = 0) -> None:
+        """Two-step launches in column tiles of blocks of at most `threads`
+        threads (a multiple of 64; 0 = whole rows up to 1024 threads), each
+        tile with a halo column on each side; cols_per_thread 1 or 2 (0: 2 for
+        FP32/FP16, FP64 always 1).  Identical results."""
+        check(_lib.load().lbm_set_sweep_tile(self._h, int(threads), int(cols_per_thread)))
+
     # -- state ------------------------------------------------------------------
     def _shape_check(self, f=None, rho=None, u=None):
         if f is not None:

@@ -165,11 +165,11 @@ def _time_run(lat, omega, steps) -> float:
 
 def autotune_lattice(lat: Lattice, omega: float, steps: int = 30) -> Tuple[int, bool, bool, int, int]:
     """Fastest launch shape (nodes per thread, XCD order, non-temporal stores,
-    steps per launch, rows per block of the two-step sweep) for a lattice of
-    this storage and size bucket, timed on a scratch lattice with the same
-    shape and boundary; applied to `lat` and cached.  Two steps per launch is
-    tried where the lattice supports it, with the occupancy-derived row count
-    (0) and with 1, 2 and 4 rows per block: lattices of a few hundred rows
+    steps per launch, rows per block of the sweep) for a lattice of this
+    storage and size bucket, timed on a scratch lattice with the same shape
+    and boundary; applied to `lat` and cached.  Two and three steps per launch
+    are tried where the lattice supports them, with the occupancy-derived row
+    count (0) and with a few rows per block: lattices of a few hundred rows
     fill the chip only with very short blocks."""
     from ._lib import LbmError
 
@@ -190,23 +190,25 @@ def autotune_lattice(lat: Lattice, omega: float, steps: int = 30) -> Tuple[int, 
             _time_run(scratch, omega, 3)
             timings.append((min(_time_run(scratch, omega, steps) for _ in range(2)), cand + (1, 0)))
         scratch.set_launch(1, True, False)
-        for rows in (0, 1, 2, 4):
+        for k, rows in ((2, 0), (2, 1), (2, 2), (2, 4), (3, 0), (3, 2), (3, 4)):
             try:
-                scratch.set_temporal(2, rows)
+                scratch.set_temporal(k, rows)
             except LbmError:
-                break
-            _time_run(scratch, omega, 4)
-            timings.append((min(_time_run(scratch, omega, steps) for _ in range(2)), (1, True, False, 2, rows)))
+                continue
+            _time_run(scratch, omega, 6)
+            timings.append((min(_time_run(scratch, omega, steps) for _ in range(2)), (1, True, False, k, rows)))
         scratch.close()
         best = min(timings)[1]
         _LATTICE_CACHE[key] = best
     lat.set_launch(*best[:3])
-    try:
-        lat.set_temporal(best[3], best[4])
-    except LbmError:
-        # a lattice of the same size bucket that the two-step kernel does not take
-        best = best[:3] + (1, 0)
-        lat.set_temporal(1)
+    # a lattice of the same size bucket may not take the cached shape: fall
+    # back to two steps, then to one
+    for k in range(best[3], 0, -1):
+        try:
+            lat.set_temporal(k, best[4] if k == best[3] else 0)
+            return best if k == best[3] else best[:3] + (k, 0)
+        except LbmError:
+            continue
     return best
 
 
