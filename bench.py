@@ -17,7 +17,12 @@ roofline   : HBM bound; achieved = algorithmic bytes per launch (every populatio
              would need for the same MLUPS (72 B per node step); copy_gbs: a
              1 GiB streaming copy (16 bytes per lane) timed the same way, the
              practical ceiling, and frac_of_copy = achieved / copy_gbs.
+             row_sweep_pattern_gbs: the two-step kernel's access pattern with no
+             work (lbm_row_sweep_probe), the ceiling of a row sweep on this
+             layout, and frac_of_row_sweep_pattern = achieved / that.
 single_step: the same workload with one step per launch (k_fused).
+three_steps_per_launch: two against three steps per launch (k_sweepk) on an
+             8192x1024 FP32 channel (three steps take rows of up to 1024).
 fp64, f16  : the same workload with FP64 populations / FP16 deviation storage,
              two steps per launch (FP64 rows of 2048 split into three column
              tiles); fp64_single_step: FP64 with one step per launch.
@@ -129,6 +134,75 @@ def copy_gbs(torch, nbytes=1 << 30, reps=10):
     del a, b
     torch.cuda.empty_cache()
     return 2 * nbytes * reps / (ms / 1e3) / 1e9
+
+
+def row_sweep_gbs(torch, rows=32, reps=10):
+    """HBM bandwidth of the two-step kernel's access pattern with no work
+    (the bench lattice's shape: nine planes of 2048 FP32 values per row, 32
+    rows per block, one row prefetched, a barrier per row), read + write
+    counted: the ceiling a row sweep reaches on this layout."""
+    import ctypes
+
+    from lattice_boltzmann_parallel_solver_amd import _lib
+
+    pitch = LY
+    a = torch.zeros(9 * LX * pitch, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    stream = torch.cuda.current_stream().cuda_stream
+    L = _lib.load()
+
+    def once():
+        _lib.check(L.lbm_row_sweep_probe(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), LX, pitch,
+                                         rows, ctypes.c_void_p(stream)))
+
+    once()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        once()
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end)
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * 9 * LX * pitch * 4 * reps / (ms / 1e3) / 1e9
+
+
+def three_step_leg(torch, steps=240, ly=1024):
+    """Two against three steps per launch on a von Karman channel of
+    LX x 1024 (plate ly/5 at x = LX/4, +1 y offset, walls, inlet U_IN, Re
+    RE): three steps per launch apply to rows of up to 1024 columns."""
+    from lattice_boltzmann_parallel_solver_amd import Lattice
+    from lattice_boltzmann_parallel_solver_amd.boundary_utils import von_karman_boundary_conditions
+
+    plate = ly // 5
+    bc = von_karman_boundary_conditions(LX, ly, 1.0, U_IN, plate, walls=True, y_offset=1)
+    lat = Lattice(LX, ly, boundary=bc, storage="f32", stream=torch.cuda.current_stream().cuda_stream)
+    rho = torch.ones((LX, ly), dtype=torch.float64, device="cuda")
+    u = torch.zeros((LX, ly, 2), dtype=torch.float64, device="cuda")
+    u[..., 0] = U_IN
+    lat.set_equilibrium(rho, u)
+    del rho, u
+    om = 1.0 / (3.0 * U_IN * plate / RE + 0.5)
+    out = {"workload": f"von Karman channel {LX}x{ly}, plate {plate}, walls, f32"}
+    for k in (2, 3):
+        lat.set_temporal(k)
+        lat.run(om, 24)
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record()
+        lat.run(om, steps)
+        end.record()
+        torch.cuda.synchronize()
+        sec = start.elapsed_time(end) / 1e3
+        out[f"mlups_{k}_steps_per_launch"] = LX * ly * steps / sec / 1e6
+        out[f"roofline_frac_{k}_steps_per_launch"] = 72 * LX * ly / (sec / (steps / k)) / 1e9 / HBM_PEAK_GBS
+    lat.close()
+    torch.cuda.empty_cache()
+    return out
 
 
 def time_steps(lat, steps, warmup, torch):
@@ -371,6 +445,8 @@ def main():
         f16["error_vs_fp64"] = f16_error_note()
 
     copy = copy_gbs(torch)
+    sweep_pattern = row_sweep_gbs(torch)
+    three = three_step_leg(torch)
     latency = None if args.no_serving else serving_leg()
     batching = None if args.no_batching else batching_leg()
 
@@ -397,8 +473,10 @@ def main():
                      "bytes_per_node_launch": BYTES_PER_NODE[args.storage],
                      "bytes_per_node_step": BYTES_PER_NODE[args.storage] // spl,
                      "single_step_equivalent_gbs": mlups * 1e6 * BYTES_PER_NODE[args.storage] / 1e9,
-                     "copy_gbs": copy, "frac_of_copy": achieved / copy},
+                     "copy_gbs": copy, "frac_of_copy": achieved / copy,
+                     "row_sweep_pattern_gbs": sweep_pattern, "frac_of_row_sweep_pattern": achieved / sweep_pattern},
         "single_step": single,
+        "three_steps_per_launch": three,
         "fp64": fp64,
         "fp64_single_step": fp64_single,
         "f16": f16,

@@ -196,6 +196,7 @@ int lbm_batch_destroy(lbm_batch *batch);
 int lbm_batch_set_launch(lbm_batch *batch, int nodes_per_thread, int use_graph);
 /* Advance every context of the batch nsteps reference steps (contexts still
  * holding a loaded state start with their first collision).  Synchronous. */
+int lbm_batch_run(lbm_batch *batch, int64_t nsteps);
 /* Two steps per launch for the batch (steps_per_launch = 2): every context
  * the two-step sweep kernel supports (no pressure fix, ny <= 2048 FP32/FP16 or
  * 1024 FP64) is advanced by it, grouped by block width, `rows_per_block` rows
@@ -222,7 +223,13 @@ int lbm_batch_set_packing(lbm_batch *batch, int packed);
  * HBM ceiling (read + write) the step kernels are measured against.  No
  * counterpart in the reference. */
 int lbm_copy_probe(const void *src, void *dst, int64_t bytes, void *stream);
-int lbm_batch_run(lbm_batch *batch, int64_t nsteps);
+/* The two-step kernel's memory access pattern with no work: FP32 buffers src
+ * and dst of 9 * nx * pitch values (LBM_DEVICE), nine planes of `pitch`
+ * values per row, each block sweeping `rows` consecutive rows, loading the
+ * next row before it writes the current one, a barrier per row; asynchronous
+ * on `stream`.  The bandwidth ceiling of a row sweep, beside the copy's.
+ * pitch even, at most 4096.  No counterpart in the reference. */
+int lbm_row_sweep_probe(const void *src, void *dst, int nx, int pitch, int rows, void *stream);
 
 /* ---- reductions that allocate nothing ----------------------------------- */
 /* Largest ncells lbm_moments_sum_noalloc accepts (174762 blocks of 256). */

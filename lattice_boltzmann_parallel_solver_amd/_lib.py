@@ -111,6 +111,7 @@ _SIGNATURES = {
     "lbm_bc_pressure": (_I, [_P, _P, _P, _P, _P]),
     "lbm_moments_sum_noalloc": (_I, [_P, _I64, _P, _P]),
     "lbm_copy_probe": (_I, [_P, _P, _I64, _P]),
+    "lbm_row_sweep_probe": (_I, [_P, _P, _I, _I, _I, _P]),
 }
 
 _lib = None

@@ -1466,6 +1466,17 @@ int lbm_copy_probe(const void *src, void *dst, int64_t bytes, void *stream) {
     return LBM_OK;
 }
 
+int lbm_row_sweep_probe(const void *src, void *dst, int nx, int pitch, int rows, void *stream) {
+    if (nx <= 0 || rows <= 0 || pitch <= 0 || pitch % 2 != 0 || pitch > 4096 || !src || !dst) {
+        set_error("lbm_row_sweep_probe: need device buffers, nx > 0, rows > 0 and an even pitch up to 4096");
+        return LBM_ERR_ARG;
+    }
+    LBM_HIP_CHECK(launch_row_sweep_probe(src, dst, (uint32_t)nx, (uint32_t)pitch, (uint32_t)rows,
+                                         static_cast<hipStream_t>(stream)),
+                  "lbm_row_sweep_probe");
+    return LBM_OK;
+}
+
 int lbm_batch_run(lbm_batch *b, int64_t nsteps) {
     if (!b) {
         set_error("lbm_batch_run: batch is NULL");

@@ -2450,6 +2450,63 @@ __global__ __launch_bounds__(kBlock) void k_copy16(const uint4 *__restrict__ src
     if (i < n) dst[i] = src[i];
 }
 
+// The two-step kernel's access pattern alone: block b sweeps rows [b R, b R +
+// R) of the row-interleaved layout (nine planes of pitch FP32 values per row),
+// loading the next row before it writes the current one, a barrier per row,
+// one multiply-add per value; blocks in XCD order.  NS float2 column slots per
+// lane (pitch / 2 <= NS blockDim.x).
+template <int NS>
+__global__ __launch_bounds__(1024) void k_row_sweep_probe(const float2 *__restrict__ src, float2 *__restrict__ dst,
+                                                          uint32_t lx, uint32_t pitch2, uint32_t rows) {
+    const uint32_t xa = xcd_block(blockIdx.x, gridDim.x) * rows;
+    const uint32_t xe = min(xa + rows, lx);
+    const uint32_t T = blockDim.x;
+    uint32_t col[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) col[k] = min(threadIdx.x + k * T, pitch2 - 1);
+    float2 cur[NS][9], nxt[NS][9];
+#pragma unroll
+    for (int k = 0; k < NS; k++)
+#pragma unroll
+        for (int i = 0; i < 9; i++) cur[k][i] = src[((size_t)xa * 9 + i) * pitch2 + col[k]];
+    for (uint32_t x = xa; x < xe; x++) {
+        if (x + 1 < xe) {
+#pragma unroll
+            for (int k = 0; k < NS; k++)
+#pragma unroll
+                for (int i = 0; i < 9; i++) nxt[k][i] = src[((size_t)(x + 1) * 9 + i) * pitch2 + col[k]];
+        }
+        // lanes past the last slot write the last one again (same values)
+#pragma unroll
+        for (int k = 0; k < NS; k++)
+#pragma unroll
+            for (int i = 0; i < 9; i++)
+                dst[((size_t)x * 9 + i) * pitch2 + col[k]] = make_float2(fmaf(cur[k][i].x, 1.0001f, 0.5f),
+                                                                         fmaf(cur[k][i].y, 0.9999f, 0.25f));
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NS; k++)
+#pragma unroll
+            for (int i = 0; i < 9; i++) cur[k][i] = nxt[k][i];
+    }
+}
+
+hipError_t launch_row_sweep_probe(const void *src, void *dst, uint32_t lx, uint32_t pitch, uint32_t rows,
+                                  hipStream_t s) {
+    if (lx == 0 || rows == 0 || pitch == 0 || pitch % 2 != 0 || pitch > 4096) return hipErrorInvalidValue;
+    const uint32_t pitch2 = pitch / 2;
+    const unsigned grid = (lx + rows - 1) / rows;
+    if (pitch2 <= 1024) {
+        const uint32_t threads = (pitch2 + 63) / 64 * 64;
+        hipLaunchKernelGGL(k_row_sweep_probe<1>, dim3(grid), dim3(threads), 0, s, static_cast<const float2 *>(src),
+                           static_cast<float2 *>(dst), lx, pitch2, rows);
+    } else {
+        hipLaunchKernelGGL(k_row_sweep_probe<2>, dim3(grid), dim3(1024), 0, s, static_cast<const float2 *>(src),
+                           static_cast<float2 *>(dst), lx, pitch2, rows);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t s) {
     if (n16 <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_copy16, dim3((unsigned)((n16 + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,

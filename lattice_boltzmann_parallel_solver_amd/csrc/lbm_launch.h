@@ -96,6 +96,10 @@ hipError_t launch_sweep2_packed(int storage, const SweepJob *jobs, const PackedS
 bool sweep_big(int storage, const Geom &g);
 // LDS bytes of the sweep kernel for a lattice of this ly, class count and rows per block.
 size_t sweep_lds_bytes(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows);
+// The two-step kernel's row-sweep access pattern with no work (measurement
+// only): FP32 buffers of 9 lx pitch values, `rows` rows per block.
+hipError_t launch_row_sweep_probe(const void *src, void *dst, uint32_t lx, uint32_t pitch, uint32_t rows,
+                                  hipStream_t s);
 // dst[i] = src[i] for n16 16-byte chunks (streaming copy, measurement only).
 hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t s);
 // *step += k (one thread).
