@@ -23,6 +23,9 @@ roofline   : HBM bound; achieved = algorithmic bytes per launch (every populatio
 single_step: the same workload with one step per launch (k_fused).
 three_steps_per_launch: two against three steps per launch (k_sweepk) on an
              8192x1024 FP32 channel (three steps take rows of up to 1024).
+other_workloads_fp64: GPU rates of the shear wave (300x300, its whole 3000
+             steps), the 4200x1800 channel and the 2048x2048 Couette and
+             Poiseuille boxes in FP64.
 fp64, f16  : the same workload with FP64 populations / FP16 deviation storage,
              two steps per launch (FP64 rows of 2048 split into three column
              tiles); fp64_single_step: FP64 with one step per launch.
@@ -201,6 +204,63 @@ def three_step_leg(torch, steps=240, ly=1024):
         out[f"mlups_{k}_steps_per_launch"] = LX * ly * steps / sec / 1e6
         out[f"roofline_frac_{k}_steps_per_launch"] = 72 * LX * ly / (sec / (steps / k)) / 1e9 / HBM_PEAK_GBS
     lat.close()
+    torch.cuda.empty_cache()
+    return out
+
+
+def config_legs(torch):
+    """Rates of the other workloads on the GPU, FP64 (the oracle's arithmetic
+    bit for bit), each lattice resident, with the launch mode it supports
+    fastest (two steps per launch except the pressure-driven box):
+    the 300x300 shear wave (omega 1, epsilon 0.05) over its whole 3000-step
+    run, the 4200x1800 von Karman channel (plate 400 at x = 1050, +1 y offset,
+    walls, inlet 0.1, Re 100), the 2048x2048 Couette box (lid 0.05, omega 1)
+    and the 2048x2048 Poiseuille box (delta p 1.5e-3, omega 1.5)."""
+    from lattice_boltzmann_parallel_solver_amd import Lattice
+    from lattice_boltzmann_parallel_solver_amd import boundary_utils as BU
+    from lattice_boltzmann_parallel_solver_amd.initial_values import sinusoidal_velocity_x
+
+    def timed(lat, om, steps):
+        lat.run(om, 4)
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record()
+        lat.run(om, steps)
+        end.record()
+        torch.cuda.synchronize()
+        return start.elapsed_time(end) / 1e3
+
+    def equilibrium_at_rest(lat, u_x=0.0):
+        rho = torch.ones((lat.lx, lat.ly), dtype=torch.float64, device="cuda")
+        u = torch.zeros((lat.lx, lat.ly, 2), dtype=torch.float64, device="cuda")
+        u[..., 0] = u_x
+        lat.set_equilibrium(rho, u)
+
+    out = {}
+    stream = torch.cuda.current_stream().cuda_stream
+    lat = Lattice(300, 300, storage="f64", stream=stream)
+    rho, u = sinusoidal_velocity_x((300, 300), 0.05)
+    lat.set_equilibrium(rho, u)
+    lat.set_temporal(2)
+    sec = timed(lat, 1.0, 3000)
+    out["shear_wave_300x300"] = {"steps": 3000, "seconds": sec, "mlups": 90000 * 3000 / sec / 1e6,
+                                 "steps_per_launch": 2, "storage": "f64"}
+    lat.close()
+    cases = (("channel_4200x1800", 4200, 1800,
+              BU.von_karman_boundary_conditions(4200, 1800, 1.0, U_IN, 400, walls=True, y_offset=1),
+              1.0 / (3.0 * U_IN * 400 / RE + 0.5), U_IN, 2),
+             ("couette_2048x2048", 2048, 2048, BU.couette_flow_boundary_conditions(2048, 2048, 0.05, 1.0), 1.0, 0.0, 2),
+             ("poiseuille_2048x2048", 2048, 2048,
+              BU.poiseuille_flow_boundary_conditions(2048, 2048, (1 + 1.5e-3) / 3, (1 - 1.5e-3) / 3), 1.5, 0.0, 1))
+    for name, lx, ly, bc, om, u_x, spl in cases:
+        lat = Lattice(lx, ly, boundary=bc, storage="f64", stream=stream)
+        equilibrium_at_rest(lat, u_x)
+        lat.set_temporal(spl)
+        sec = timed(lat, om, 200)
+        out[name] = {"steps": 200, "seconds": sec, "mlups": lx * ly * 200 / sec / 1e6, "steps_per_launch": spl,
+                     "storage": "f64"}
+        lat.close()
     torch.cuda.empty_cache()
     return out
 
@@ -447,6 +507,7 @@ def main():
     copy = copy_gbs(torch)
     sweep_pattern = row_sweep_gbs(torch)
     three = three_step_leg(torch)
+    configs = config_legs(torch)
     latency = None if args.no_serving else serving_leg()
     batching = None if args.no_batching else batching_leg()
 
@@ -477,6 +538,7 @@ def main():
                      "row_sweep_pattern_gbs": sweep_pattern, "frac_of_row_sweep_pattern": achieved / sweep_pattern},
         "single_step": single,
         "three_steps_per_launch": three,
+        "other_workloads_fp64": configs,
         "fp64": fp64,
         "fp64_single_step": fp64_single,
         "f16": f16,

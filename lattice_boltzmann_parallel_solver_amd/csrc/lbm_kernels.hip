@@ -1391,6 +1391,7 @@ struct SweepKParams {
     int64_t probe_cap;
     int64_t probe_slot;
     uint32_t rows;        // output rows per block
+    int nt;               // non-temporal stores
     S *scratch;           // 9 blockDim.x values, written and never read
 };
 
@@ -1648,7 +1649,7 @@ __device__ __forceinline__ void sweepk_block(const SweepKParams<S> &p, uint32_t 
                     // values into the row padding (never read)
                     const uint32_t yst = min(tid, pitch - 1);
 #pragma unroll
-                    for (int d = 0; d < 9; d++) Io<S>::st(pop32<kBig>(gdst, pitch, r, d, yst), o2.s[d], false);
+                    for (int d = 0; d < 9; d++) Io<S>::st(pop32<kBig>(gdst, pitch, r, d, yst), o2.s[d], p.nt != 0);
                     if ((flags2 & kFlagOutletSide) && tid < ly) {
                         gside[(size_t)y * 3 + 0] = w2[3];
                         gside[(size_t)y * 3 + 1] = w2[6];
@@ -2285,6 +2286,7 @@ hipError_t sweepk_typed(int K, const Geom &g, const StateView &st, int src, cons
     p.probe_ring = pr.ring;
     p.probe_cap = pr.capacity;
     p.probe_slot = slot;
+    p.nt = cfg.nt_stores;
     p.scratch = static_cast<S *>(st.scratch);
     const unsigned threads = (g.ly + 63) / 64 * 64;
     if (!p.scratch || (size_t)9 * threads * sizeof(S) > kSweepScratchBytes) return hipErrorInvalidValue;
