@@ -849,6 +849,13 @@ void release_batch(lbm_batch *b) {
     if (b->h_stage) (void)hipHostFree(b->h_stage);
 }
 
+// Whether three steps per launch beat two for FP32 rows of `waves`
+// wavefronts (measured on 8192-row channels: blocks of 7-8 and of 11 or more
+// waves gain 10-36 %; 9-10 lose or tie, 6 and fewer lose or tie).
+bool three_steps_pay(int storage, unsigned waves) {
+    return storage == LBM_STORE_F32 && (waves == 7 || waves == 8 || waves >= 11);
+}
+
 // Side streams of a batch, and the batch size (nodes) from which its groups
 // run on them: below it a pair of launches is short and the graph replay
 // on one stream costs less than the extra launches.
@@ -886,7 +893,9 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
         int v = 1;
         unsigned threads = 0;
         bool big = false;  // state buffers of 4 GiB or more
-        size_t lds = 0;
+        int k = 2;         // 3: three steps per launch where the group runs on a side stream
+        size_t lds = 0;    // of the two-step kernel
+        size_t lds_k = 0;  // of the three-step kernel
         std::vector<int> members;
         unsigned nblocks = 0;  // blocks of the group's launch
     };
@@ -908,15 +917,20 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             pjobs.push_back(j);
             continue;
         }
+        const int k = b->temporal == 3 && v == 1 && !big && three_steps_pay(c->storage, threads / 64) &&
+                              sweepk_supported(c->storage, c->g, c->bc.view, 3)
+                          ? 3
+                          : 2;
         Bucket *bk = nullptr;
         for (Bucket &x : buckets)
-            if (x.v == v && x.threads == threads && x.big == big) bk = &x;
+            if (x.v == v && x.threads == threads && x.big == big && x.k == k) bk = &x;
         if (!bk) {
             buckets.emplace_back();
             bk = &buckets.back();
             bk->v = v;
             bk->threads = threads;
             bk->big = big;
+            bk->k = k;
         }
         bk->members.push_back(j);
     }
@@ -932,12 +946,21 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             // few rows per block; wide rows (512+ threads per block) take 32
             LaunchCfg cfg = c->cfg;
             cfg.sweep_rows = bk.threads >= 512 ? std::max(b->sweep_rows, 32) : b->sweep_rows;
-            const uint32_t rows = (uint32_t)sweep_rows_for(c->g, cfg, 0);
+            // three steps per launch read three halo rows on each side: four
+            // times the rows per block (the 256-channel batch with its
+            // three-step groups at 8, 16, 24, 32 and 48 rows: 132, 143, 147,
+            // 149 and 148 GLUPS)
+            if (bk.k == 3) cfg.sweep_rows = std::max(cfg.sweep_rows, 4 * b->sweep_rows);
+            uint32_t rows = (uint32_t)sweep_rows_for(c->g, cfg, 0);
+            // the three-step kernel needs two rows in every block; the same
+            // tables also serve its two-step launches
+            if (bk.k == 3) rows = (uint32_t)sweepk_rows(c->g, (int)rows, 3);
             const uint32_t nb = (c->g.lx + rows - 1) / rows;
             const uint32_t block0 = (uint32_t)t.block_job.size();
             t.block_job.insert(t.block_job.end(), nb, (uint32_t)m);
-            bk.lds = std::max(bk.lds, sweep_lds_bytes(c->storage, c->g.ly, c->bc.view.chunk ? c->bc.view.nclasses : 0,
-                                                      rows));
+            const uint32_t ncl = c->bc.view.chunk ? c->bc.view.nclasses : 0;
+            bk.lds = std::max(bk.lds, sweep_lds_bytes(c->storage, c->g.ly, ncl, rows));
+            if (bk.k == 3) bk.lds_k = std::max(bk.lds_k, sweepk_lds_bytes(c->storage, c->g.ly, ncl, rows, 3));
             for (int pp = 0; pp < 2; pp++) {
                 SweepJob d;
                 const int src = c->cur ^ pp;
@@ -1046,8 +1069,11 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
         LBM_HIP_CHECK(packed_grid(b->storage, plds, npblocks, b->packed == 2, &pgrid), "lbm_batch_run: packed grid");
     pack_batch_tables(gt, pt, st, b->image);
     const TableImage &img = b->image;
-    for (const Bucket &bk : buckets)
+    for (const Bucket &bk : buckets) {
         LBM_HIP_CHECK(prepare_sweep2_batch(b->storage, bk.v, bk.big, bk.lds), "lbm_batch_run: sweep kernel LDS");
+        if (bk.k == 3)
+            LBM_HIP_CHECK(prepare_sweepk_batch(b->storage, 3, bk.lds_k), "lbm_batch_run: three-step kernel LDS");
+    }
     drop_graph(b);  // the one-step graph of an earlier run would hold stale tables
     if (img.bytes.size() > b->sweep_cap) {
         (void)hipFree(b->d_sweep);
@@ -1087,6 +1113,10 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     };
     const int64_t pairs = nsteps / 2;
     int64_t done_pairs = 0;
+    // groups of three steps per launch advance the pairs' 2 * pairs steps in
+    // q launches and a remainder of r steps (side streams only)
+    const int64_t q3 = 2 * pairs / 3, r3 = 2 * pairs - 3 * q3;
+    bool three_ran = false;
     // A large batch of several groups: each group (and the one-step jobs) on
     // a side stream of its own, so that the tail of one group's launch
     // overlaps the next group instead of idling the GPU (one stream runs its
@@ -1128,10 +1158,39 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
         }
         LBM_HIP_CHECK(hipEventRecord(b->fork, b->stream), "lbm_batch_run: fork");
         for (size_t l = 0; l < nl; l++) LBM_HIP_CHECK(hipStreamWaitEvent(b->lanes[l], b->fork, 0), "lbm_batch_run: fork");
+        for (size_t i = 0; i < buckets.size(); i++) {
+            // a three-step group: its own sequence of launches on its lane
+            const Bucket &bk = buckets[i];
+            if (bk.k != 3) continue;
+            hipStream_t lane = b->lanes[lane_of[i]];
+            for (int64_t t = 0; t < q3; t++)
+                LBM_HIP_CHECK(launch_sweepk_batch(b->storage, 3,
+                                                  reinterpret_cast<const SweepJob *>(dev + img.group_jobs[t & 1][i]),
+                                                  reinterpret_cast<const uint32_t *>(dev + img.group_blocks[i]),
+                                                  bk.nblocks, bk.threads, bk.lds_k, nullptr, b->step_host + 3 * t, lane),
+                              "batched three-step sweep");
+            if (r3 == 2) {
+                LBM_HIP_CHECK(launch_sweep2_batch(b->storage, bk.v, bk.big,
+                                                  reinterpret_cast<const SweepJob *>(dev + img.group_jobs[q3 & 1][i]),
+                                                  reinterpret_cast<const uint32_t *>(dev + img.group_blocks[i]),
+                                                  bk.nblocks, bk.threads, bk.lds, nullptr, b->step_host + 3 * q3, lane),
+                              "batched sweep after three-step launches");
+            } else if (r3 == 1) {
+                for (int j : bk.members) {
+                    lbm_ctx *c = b->ctxs[j];
+                    const int64_t slot = c->probe.cell >= 0 ? c->steps + 3 * q3 - c->probe_base - 1 : -1;
+                    LBM_HIP_CHECK(launch_fused(c->storage, c->g, c->st, c->cur ^ (int)(q3 & 1), c->bc.view, b->omegas[j],
+                                               c->probe, slot, nullptr, c->cfg, lane, nullptr),
+                                  "step after three-step launches");
+                }
+            }
+        }
+        three_ran = true;
         for (int64_t k = 0; k < pairs; k++) {
             const int pp = (int)(k & 1);
             for (size_t i = 0; i < buckets.size(); i++) {
                 const Bucket &bk = buckets[i];
+                if (bk.k == 3) continue;
                 LBM_HIP_CHECK(launch_sweep2_batch(b->storage, bk.v, bk.big,
                                                   reinterpret_cast<const SweepJob *>(dev + img.group_jobs[pp][i]),
                                                   reinterpret_cast<const uint32_t *>(dev + img.group_blocks[i]),
@@ -1183,8 +1242,12 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     }
     for (; done_pairs < pairs; done_pairs++) LBM_HIP_CHECK(pair((int)(done_pairs & 1)), "batched sweep pair");
     b->step_host += 2 * pairs;
-    for (const Bucket &bk : buckets)
-        for (int j : bk.members) b->ctxs[j]->cur ^= (int)(pairs & 1);
+    for (const Bucket &bk : buckets) {
+        // launches that flipped each member's buffers: one per pair, or per
+        // three-step launch plus the remainder's one
+        const int64_t flips = bk.k == 3 && three_ran ? q3 + (r3 > 0 ? 1 : 0) : pairs;
+        for (int j : bk.members) b->ctxs[j]->cur ^= (int)(flips & 1);
+    }
     for (int j : pjobs) b->ctxs[j]->cur ^= (int)(pairs & 1);
     for (int j = 0; j < n; j++) b->ctxs[j]->steps += 2 * pairs;
     if (nsteps & 1) {
@@ -1287,8 +1350,8 @@ int lbm_batch_set_launch(lbm_batch *b, int v, int use_graph) {
 }
 
 int lbm_batch_set_temporal(lbm_batch *b, int steps_per_launch, int rows_per_block) {
-    if (!b || (steps_per_launch != 1 && steps_per_launch != 2) || rows_per_block < 0) {
-        set_error("lbm_batch_set_temporal: steps_per_launch must be 1 or 2 and rows_per_block >= 0");
+    if (!b || steps_per_launch < 1 || steps_per_launch > 3 || rows_per_block < 0) {
+        set_error("lbm_batch_set_temporal: steps_per_launch must be 1, 2 or 3 and rows_per_block >= 0");
         return LBM_ERR_ARG;
     }
     b->temporal = steps_per_launch;
@@ -1339,7 +1402,7 @@ int batch_run_body(lbm_batch *b, int64_t nsteps) {
         }
         nsteps--;
     }
-    if (b->temporal == 2) {
+    if (b->temporal >= 2) {
         // Two steps per launch pays when the jobs it cannot take (the pressure
         // fix) are few: their one-step launches run after the groups and only
         // fill the GPU with their own blocks.  Above 15 % of the nodes the whole

@@ -1728,6 +1728,36 @@ __global__ __launch_bounds__(kSweepThreads, K == 3 && !kBig ? 5 : 4) void k_swee
     sweepk_block<S, K, kBig>(p, blk * p.rows, smem);
 }
 
+// K steps of every job of a batch bucket (state buffers below 4 GiB): block b
+// sweeps rows of job block_job[b]; the samples of step s go to ring slot
+// (*step or step_now) + probe_offset + s - 1.
+template <class S, int K>
+__global__ __launch_bounds__(kSweepThreads, K == 3 ? 5 : 4) void k_sweepk_batch(const SweepJob *__restrict__ jobs,
+                                                                               const uint32_t *__restrict__ block_job,
+                                                                               const int64_t *__restrict__ step,
+                                                                               int64_t step_now) {
+    using C = typename Io<S>::C;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const SweepJob &j = jobs[block_job[blk]];
+    SweepKParams<S> p;
+    p.src = static_cast<const S *>(j.src);
+    p.dst = static_cast<S *>(j.dst);
+    p.side_prev = static_cast<const C *>(j.side_prev);
+    p.side_next = static_cast<C *>(j.side_next);
+    p.g = j.g;
+    p.bc = j.bc;
+    p.omega = j.omega;
+    p.probe_cell = j.probe_cell;
+    p.probe_ring = j.probe_ring;
+    p.probe_cap = j.probe_cap;
+    p.probe_slot = (step ? *step : step_now) + j.probe_offset;
+    p.rows = j.rows;
+    p.nt = 0;
+    p.scratch = static_cast<S *>(j.scratch);
+    sweepk_block<S, K, false>(p, (blk - j.block0) * j.rows, smem);
+}
+
 __global__ void k_add_step(int64_t *step, int64_t k) { *step += k; }
 
 // ---------------------------------------------------------------------------
@@ -2314,6 +2344,36 @@ hipError_t sweepk_typed(int K, const Geom &g, const StateView &st, int src, cons
     return e != hipSuccess ? e : hipGetLastError();
 }
 }  // namespace
+
+size_t sweepk_lds_bytes(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows, int K) {
+    return sweepk_lds_for(storage, ly, nclasses, rows, K);
+}
+
+namespace {
+const void *sweepk_batch_fn(int storage, int K) {
+    if (K != 3) return nullptr;
+    if (storage == LBM_STORE_F32) return reinterpret_cast<const void *>(k_sweepk_batch<float, 3>);
+    if (storage == LBM_STORE_F16) return reinterpret_cast<const void *>(k_sweepk_batch<__half, 3>);
+    return nullptr;
+}
+}  // namespace
+
+hipError_t prepare_sweepk_batch(int storage, int K, size_t lds) {
+    const void *fn = sweepk_batch_fn(storage, K);
+    if (!fn || lds > kLdsLimit) return hipErrorInvalidValue;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+hipError_t launch_sweepk_batch(int storage, int K, const SweepJob *jobs, const uint32_t *block_job, unsigned nblocks,
+                               unsigned threads, size_t lds, const int64_t *step, int64_t step_now, hipStream_t s) {
+    if (nblocks == 0) return hipSuccess;
+    const void *fn = sweepk_batch_fn(storage, K);
+    if (!fn || threads == 0 || threads > (unsigned)kSweepThreads || threads % 64 != 0 || lds > kLdsLimit)
+        return hipErrorInvalidValue;
+    void *args[] = {&jobs, &block_job, &step, &step_now};
+    hipError_t e = hipLaunchKernel(fn, dim3(nblocks), dim3(threads), args, lds, s);
+    return e != hipSuccess ? e : hipGetLastError();
+}
 
 bool sweepk_supported(int storage, const Geom &g, const BcView &bc, int K) {
     if (K < 3 || K > kSweepMaxK) return false;

@@ -47,6 +47,14 @@ bool sweepk_supported(int storage, const Geom &g, const BcView &bc, int K);
 // Rows per block of the K-step kernel: `rows`, raised until every block
 // (the last one included) holds at least K - 1 rows.
 int sweepk_rows(const Geom &g, int rows, int K);
+// LDS bytes of the K-step kernel for a lattice of this ly, class count and rows per block.
+size_t sweepk_lds_bytes(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows, int K);
+// K (= 3) steps of every job of a batch bucket (whole rows of at most 1024
+// columns, FP32/FP16, state buffers below 4 GiB, rows per block from
+// sweepk_rows); probe slots as launch_sweep2_batch, plus s - 1 for step s.
+hipError_t launch_sweepk_batch(int storage, int K, const SweepJob *jobs, const uint32_t *block_job, unsigned nblocks,
+                               unsigned threads, size_t lds, const int64_t *step, int64_t step_now, hipStream_t s);
+hipError_t prepare_sweepk_batch(int storage, int K, size_t lds);
 // The batched sweep takes whole rows only (no column tiles).
 bool sweep_batch_supported(int storage, const Geom &g, const BcView &bc);
 SweepShape sweep_shape(int storage, uint32_t ly, int tile_threads = 0, int tile_v = 0);

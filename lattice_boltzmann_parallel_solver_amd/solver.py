@@ -332,7 +332,9 @@ class Batch:
 
     def set_temporal(self, steps_per_launch: int = 2, rows_per_block: int = 0) -> None:
         """Two steps per launch for every lattice the sweep kernel supports
-        (identical results); the others keep one step per launch."""
+        (identical results); the others keep one step per launch.  3: in large
+        batches, FP32 lattices of 385-512 or 641-1024 columns take three steps
+        per launch, the rest as with 2."""
         check(_lib.load().lbm_batch_set_temporal(self._h, int(steps_per_launch), int(rows_per_block)))
 
     def set_packing(self, packed: int = 0) -> None:

@@ -265,16 +265,21 @@ def test_batched_sweep_edge_widths_equal_individual_runs(storage):
         lat.close()
 
 
-def test_large_batch_groups_on_side_streams_equal_individual_runs():
+@pytest.mark.parametrize("temporal", [2, 3])
+def test_large_batch_groups_on_side_streams_equal_individual_runs(temporal):
     """A batch above 8 Mi nodes with several groups and a pressure-fix job runs
     its groups on side streams; states and probe traces still equal each
-    lattice's own single-step run, over several runs of odd and even length."""
+    lattice's own single-step run, over several runs of odd and even length.
+    With three steps per launch the groups of 8 and of 15-16 waves per row
+    take the three-step kernel, whose runs end with a two-step launch or a
+    single step where the run's length is not a multiple of three."""
     from lattice_boltzmann_parallel_solver_amd import Batch, Lattice
     from lattice_boltzmann_parallel_solver_amd.boundary_utils import (poiseuille_flow_boundary_conditions,
                                                                       von_karman_boundary_conditions)
     from lattice_boltzmann_parallel_solver_amd.initial_values import density_1_velocity_x_u0_velocity_y_0_initial
 
-    shapes = [(2048, 2048), (3000, 1000), (1600, 900), (700, 300), (900, 300)]
+    shapes = [(2048, 2048), (3000, 1000), (1600, 900), (700, 300), (900, 300), (1100, 480)]
+    chunks = (6, 7, 4) if temporal == 2 else (6, 7, 4, 8)
 
     def build():
         lats, oms = [], []
@@ -297,8 +302,8 @@ def test_large_batch_groups_on_side_streams_equal_individual_runs():
     lats, oms = build()
     assert sum(l.lx * l.ly for l in lats) >= 8 << 20
     ref, _ = build()
-    b = Batch(lats, oms, temporal=2)
-    for chunk in (6, 7, 4):
+    b = Batch(lats, oms, temporal=temporal)
+    for chunk in chunks:
         b.run(chunk)
         for r, om in zip(ref, oms):
             r.run(om, chunk)
@@ -307,14 +312,15 @@ def test_large_batch_groups_on_side_streams_equal_individual_runs():
         for a, c in zip(lat.state(), r.state()):
             assert np.array_equal(a, c), (lat.lx, lat.ly)
         assert np.array_equal(lat.probe(), r.probe()), (lat.lx, lat.ly)
-    # against the FP64 oracle (17 steps)
+    # against the FP64 oracle
+    total = sum(chunks)
     want = []
     for (lx, ly), om in zip(shapes, oms):
         rho, u = O.density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), 0.1)
         bc = O.von_karman_boundary_conditions(lx, ly, 1.0, 0.1, max(2, ly // 5), walls=True, y_offset=1)
-        want.append(N.run(O.equilibrium(rho, u), rho, u, om, 17, bc, probe=(3 * lx // 4, ly // 2)))
+        want.append(N.run(O.equilibrium(rho, u), rho, u, om, total, bc, probe=(3 * lx // 4, ly // 2)))
     rho, u = O.density_1_velocity_0_initial((64, 32))
-    want.append(N.run(O.equilibrium(rho, u), rho, u, 1.5, 17,
+    want.append(N.run(O.equilibrium(rho, u), rho, u, 1.5, total,
                       O.poiseuille_flow_boundary_conditions(64, 32, 1.0015 / 3, 0.9985 / 3), probe=(32, 16)))
     check_oracle(lats, want, "f32", probes=True)
     for lat in lats + ref:
