@@ -444,6 +444,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the three-step leg and the other workloads' rates (profiling the main legs)")
+    ap.add_argument("--three-step-only", action="store_true",
+                    help="only the two- against three-step leg on the 8192x1024 channel (profiling k_sweepk)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -451,13 +455,16 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the lattice is not split across devices)", file=sys.stderr)
         sys.exit(2)
-    cpu = None if (args.no_cpu_baseline or args.copy_calibration) else cpu_baseline()
+    cpu = None if (args.no_cpu_baseline or args.copy_calibration or args.three_step_only) else cpu_baseline()
 
     import torch
 
     if not torch.cuda.is_available():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.three_step_only:
+        print(json.dumps({"three_steps_per_launch": three_step_leg(torch)}))
+        return
     if args.copy_calibration:
         a = torch.empty(args.copy_calibration // 4, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
@@ -507,8 +514,8 @@ def main():
 
     copy = copy_gbs(torch)
     sweep_pattern = row_sweep_gbs(torch)
-    three = three_step_leg(torch)
-    configs = config_legs(torch)
+    three = None if args.no_extra else three_step_leg(torch)
+    configs = None if args.no_extra else config_legs(torch)
     latency = None if args.no_serving else serving_leg()
     batching = None if args.no_batching else batching_leg()
 
