@@ -1257,8 +1257,12 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep2(SweepParams<S> p) {
 // Two steps of every job of a batch bucket: block b sweeps rows of job
 // block_job[b]; probe samples go to ring slots *step + offset (+1).  kBig:
 // the bucket's state buffers are 4 GiB or more (64-bit row bases).
+// One column per thread with 32-bit offsets fits 64 registers in FP32 and
+// FP16 without spilling: eight waves per SIMD, so that more of a batch's small
+// blocks share a CU (+0.5 to +2.4 % on the 256-channel batch with three-step
+// groups beside them; FP64 would spill about 115 registers).
 template <class S, int V, bool kBig>
-__global__ __launch_bounds__(kSweepThreads) void k_sweep2_batch(const SweepJob *__restrict__ jobs,
+__global__ __launch_bounds__(kSweepThreads, sizeof(S) < 8 && V == 1 && !kBig ? 8 : 4) void k_sweep2_batch(const SweepJob *__restrict__ jobs,
                                                                 const uint32_t *__restrict__ block_job,
                                                                 const int64_t *__restrict__ step, int64_t step_now) {
     using C = typename Io<S>::C;
