@@ -6,7 +6,10 @@ offset, inlet equilibrium u = (0.1, 0), rho = 1, outlet extrapolation, rigid
 top and bottom walls, nu = 0.1*400/100 (Re = 100) so omega = 1/1.7.
 
 value      : MLUPS of the FP32-deviation step (inputs resident in HBM), two steps
-             per launch (the row-sweep kernel keeps the intermediate step on chip).
+             per launch (the row-sweep kernel keeps the intermediate step on chip),
+             at the launch shape the library's autotuner picks for this lattice
+             (rows per block of the sweep; config.launch_shape), tuned outside
+             the timed region; --no-autotune times the default shape.
 roofline   : HBM bound; achieved = algorithmic bytes per launch (every population
              read and written once: 18 accesses x 4 B per node) / average launch
              time over the timed region (HIP events on the stream the kernels run
@@ -444,6 +447,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the headline lattice at the default launch shape instead of the autotuned one")
+    ap.add_argument("--rows", type=int, default=0,
+                    help="rows per block of the headline's two-step sweep (implies --no-autotune)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the three-step leg and the other workloads' rates (profiling the main legs)")
     ap.add_argument("--three-step-only", action="store_true",
@@ -475,9 +482,23 @@ def main():
         return
     N = LX * LY
 
-    steps = args.steps + args.steps % 2  # whole launches of the two-step kernel
     lat = build_lattice(args.storage, torch, args.temporal)
+    tuned = None
+    if args.rows > 0 and lat.steps_per_launch == 2:
+        lat.set_temporal(2, args.rows)
+        tuned = {"steps_per_launch": 2, "rows_per_block": args.rows}
+    elif not args.no_autotune and args.temporal == 2:
+        # the library's autotuner (launch shape, steps per launch and rows per
+        # block timed on a scratch lattice of the same shape), outside the timed region
+        from lattice_boltzmann_parallel_solver_amd import workloads as W
+
+        best = W.autotune_lattice(lat, omega(), steps=60)
+        lat.steps_per_launch = best[3]
+        tuned = {"nodes_per_thread": best[0], "xcd_order": best[1], "nt_stores": best[2],
+                 "steps_per_launch": best[3], "rows_per_block": best[4]}
+        torch.cuda.empty_cache()
     spl = lat.steps_per_launch
+    steps = -(-args.steps // spl) * spl  # whole launches
     ms = time_steps(lat, steps, args.warmup, torch)
     m, jx, _ = lat.moments_sum()
     assert m == m and abs(m / N - 1.0) < 0.1, "unstable run"
@@ -535,6 +556,7 @@ def main():
         "config": {"workload": f"von Karman channel {LX}x{LY}, plate {PLATE} at x={LX // 4} (+1 y offset), "
                                f"rigid walls, inlet u={U_IN}, Re={RE:g}, {args.storage} population storage",
                    "lx": LX, "ly": LY, "storage": args.storage, "omega": omega(), "steps_per_launch": spl,
+                   "launch_shape": tuned or "default",
                    "parallelism": "single GPU (whole lattice resident)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.storage, spl),
