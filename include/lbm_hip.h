@@ -202,10 +202,10 @@ int lbm_batch_run(lbm_batch *batch, int64_t nsteps);
  * 1024 FP64) is advanced by it, grouped by block width, `rows_per_block` rows
  * per block (0 = 8; at least 32 for lattices wide enough for blocks of 512 or
  * more threads); the others keep the one-step batched kernel.
- * steps_per_launch = 3: as 2, and in batches whose groups run on side streams
- * (8 Mi nodes or more, several groups) the FP32 groups whose rows take 7-8 or
- * 11-16 wavefronts (385-512 or 641-1024 columns) take three steps per launch,
- * where that measured faster.  Results are identical in every mode.  Default
+ * steps_per_launch = 3: as 2, and in batches of 8 Mi nodes or more the FP32
+ * groups whose rows take 7-8 or 11-16 wavefronts (385-512 or 641-1024
+ * columns) take three steps per launch, where that measured faster, each
+ * group on a side stream.  Results are identical in every mode.  Default
  * 1.  No counterpart in the reference. */
 int lbm_batch_set_temporal(lbm_batch *batch, int steps_per_launch, int rows_per_block);
 /* With two steps per launch, FP32/FP16 contexts of at most 1024 columns can

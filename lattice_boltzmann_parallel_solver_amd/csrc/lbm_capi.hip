@@ -1128,7 +1128,12 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     // groups: the buckets, then the packed launch, then the one-step jobs
     const size_t gpacked = buckets.size(), gstep = buckets.size() + (npblocks ? 1 : 0);
     const size_t groups = gstep + (fjobs.empty() ? 0 : 1);
-    if (groups > 1 && nodes >= (int64_t)kBatchLaneNodes && pairs >= 1) {
+    // three-step groups run only on side streams (their launch count differs
+    // from the pairs'), so a large batch holding one takes them even with a
+    // single group
+    bool any_three = false;
+    for (const Bucket &bk : buckets) any_three |= bk.k == 3;
+    if ((groups > 1 || any_three) && nodes >= (int64_t)kBatchLaneNodes && pairs >= 1) {
         const size_t nl = std::min(groups, (size_t)kBatchLanes);
         LBM_HIP_CHECK(ensure_lanes(b, nl), "lbm_batch_run: side streams");
         // groups onto lanes, largest first onto the least loaded

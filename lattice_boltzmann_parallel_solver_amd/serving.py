@@ -202,7 +202,7 @@ class Server:
     batch slot are freed for the next job."""
 
     def __init__(self, hbm_budget: int = 160 << 30, chunk_steps: int = 2000, max_active: int = 256,
-                 device: int = 0, nodes_per_thread: int = 1, temporal: int = 2, progress=None,
+                 device: int = 0, nodes_per_thread: int = 1, temporal: int = 3, progress=None,
                  policy: str = "srpt", fill_nodes: int = 96 << 20, on_result=None, mass_tolerance: float = 0.1,
                  max_mean_speed: float = 0.5, poll_s: float = 0.002):
         if policy not in ("fifo", "srpt"):
@@ -214,7 +214,10 @@ class Server:
         self.max_active = int(max_active)
         self.device = device
         self.nodes_per_thread = nodes_per_thread
-        self.temporal = temporal  # steps per launch of the batched step (2: sweep kernel where it applies)
+        # steps per launch of the batched step: 2, the sweep kernel where it
+        # applies; 3 (default), also the three-step kernel for the FP32 widths
+        # where it pays in large batches
+        self.temporal = temporal
         self.progress = progress  # called after every chunk with (seconds, active, waiting, finished)
         # called as on_result(record, lattice) when a job finishes, before its
         # lattice is freed (e.g. to read the final fields); not called for
