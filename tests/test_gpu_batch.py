@@ -327,6 +327,47 @@ def test_large_batch_groups_on_side_streams_equal_individual_runs(temporal):
         lat.close()
 
 
+def test_large_single_group_batch_with_three_steps_equals_individual_runs():
+    """A batch above 8 Mi nodes of one width that three steps per launch pay
+    for (480 columns: 8 waves) runs it on a side stream even as a single
+    group; states and probes equal single-step runs over runs whose length
+    leaves no, one and two steps after the three-step launches."""
+    from lattice_boltzmann_parallel_solver_amd import Batch, Lattice
+    from lattice_boltzmann_parallel_solver_amd.boundary_utils import von_karman_boundary_conditions
+    from lattice_boltzmann_parallel_solver_amd.initial_values import density_1_velocity_x_u0_velocity_y_0_initial
+
+    lx, ly, n = 1100, 480, 17
+
+    def build():
+        lats, oms = [], []
+        for k in range(n):
+            d = 40 + 5 * k
+            lat = Lattice(lx, ly, boundary=von_karman_boundary_conditions(lx, ly, 1.0, 0.1, d, walls=True, y_offset=1),
+                          storage="f32")
+            rho, u = density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), 0.1)
+            lat.set_equilibrium(rho, u)
+            lat.set_probe(3 * lx // 4 - k, ly // 2, 64)
+            lats.append(lat)
+            oms.append(1.0 / (3 * 0.1 * d / 100.0 + 0.5))
+        return lats, oms
+
+    lats, oms = build()
+    assert sum(l.lx * l.ly for l in lats) >= 8 << 20
+    ref, _ = build()
+    b = Batch(lats, oms, temporal=3)
+    for chunk in (7, 4, 8, 3):  # after the aligning first step: 6, 4, 8, 3 (+1 odd) steps
+        b.run(chunk)
+        for r, om in zip(ref, oms):
+            r.run(om, chunk)
+    b.close()
+    for lat, r in zip(lats, ref):
+        for a, c in zip(lat.state(), r.state()):
+            assert np.array_equal(a, c)
+        assert np.array_equal(lat.probe(), r.probe())
+    for lat in lats + ref:
+        lat.close()
+
+
 @pytest.mark.parametrize("storage,rows", [("f32", 8), ("f32", 3), ("f16", 5)])
 def test_packed_launch_equals_grouped_launches(storage, rows):
     """One packed launch per step pair (row ranges of many widths side by side
