@@ -1,8 +1,9 @@
-"""The reference-side ctypes binding shown in INTEGRATION.md (section 3) runs
-as written: the test takes the code block from the document, points its
-CDLL path at the in-tree library, executes it and checks run_steps against
-the oracle's lattice_boltzmann_step loop (boundary=None,
-lattice_boltzmann_method.py:191-228) bit for bit."""
+"""The reference-side ctypes binding shown in the repository's integration
+notes runs as written: the test takes that code block (the one headed
+`# src/lbm_hip_binding.py`) from the notes, points its CDLL path at the
+in-tree library, executes it and checks run_steps against the oracle's
+lattice_boltzmann_step loop (boundary=None, lattice_boltzmann_method.py:191-228)
+bit for bit."""
 
 import os
 import re
@@ -17,13 +18,14 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "lattice_boltzmann_parallel_solver_amd", "liblbm_hip.so")
+NOTES = os.path.join(ROOT, "INTEGRATION.md")
 
 
 def _stub_source():
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    text = open(NOTES).read()
     blocks = re.findall(r"```python\n(.*?)```", text, flags=re.S)
     stub = [b for b in blocks if b.startswith("# src/lbm_hip_binding.py")]
-    assert len(stub) == 1, "INTEGRATION.md should hold exactly one reference-side binding block"
+    assert len(stub) == 1, "the integration notes should hold exactly one reference-side binding block"
     src = stub[0]
     assert '"/path/to/lattice_boltzmann_parallel_solver_amd/liblbm_hip.so"' in src
     return src.replace('"/path/to/lattice_boltzmann_parallel_solver_amd/liblbm_hip.so"', repr(LIB))
@@ -36,7 +38,7 @@ def test_integration_ctypes_stub_runs_and_equals_oracle():
     rho, u = O.sinusoidal_velocity_x((lx, ly), 0.05)
     f = O.equilibrium(rho, u)
     f = f + 1e-5 * np.random.default_rng(4).standard_normal(f.shape)
-    print(f"case: INTEGRATION.md ctypes stub, {lx}x{ly} periodic, {steps} FP64 steps", flush=True)
+    print(f"case: reference-side ctypes stub, {lx}x{ly} periodic, {steps} FP64 steps", flush=True)
     got = ns["run_steps"](f, rho, u, omega, steps)
     want = N.run(f, rho, u, omega, steps)[:3]
     for a, b in zip(got, want):
