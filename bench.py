@@ -23,7 +23,9 @@ roofline   : HBM bound; achieved = algorithmic bytes per launch (every populatio
              row_sweep_pattern_gbs: the two-step kernel's access pattern with no
              work (lbm_row_sweep_probe), the ceiling of a row sweep on this
              layout, and frac_of_row_sweep_pattern = achieved / that.
-single_step: the same workload with one step per launch (k_fused).
+single_step: the same workload with one step per launch (k_fused); its
+             periodic_lattice entry times the same kernel on a fully periodic
+             8192x2048 lattice (no boundary nodes).
 three_steps_per_launch: two against three steps per launch (k_sweepk) on an
              8192x1024 FP32 channel (three steps take rows of up to 1024).
 other_workloads_fp64: GPU rates of the shear wave (300x300, its whole 3000
@@ -266,6 +268,27 @@ def config_legs(torch):
         lat.close()
     torch.cuda.empty_cache()
     return out
+
+
+def periodic_single_step_leg(torch, storage="f32", steps=300):
+    """The one-step kernel on a fully periodic 8192x2048 lattice (no boundary
+    nodes; uniform flow u = (0.05, 0)), the same launch shape: the fused
+    stream+collide step alone, beside the channel's legs."""
+    from lattice_boltzmann_parallel_solver_amd import Lattice
+
+    lat = Lattice(LX, LY, storage=storage, stream=torch.cuda.current_stream().cuda_stream)
+    rho = torch.ones((LX, LY), dtype=torch.float64, device="cuda")
+    u = torch.zeros((LX, LY, 2), dtype=torch.float64, device="cuda")
+    u[..., 0] = 0.05
+    lat.set_equilibrium(rho, u)
+    del rho, u
+    ms = time_steps(lat, steps, 30, torch)
+    lat.close()
+    torch.cuda.empty_cache()
+    ach = BYTES_PER_NODE[storage] * LX * LY / (ms / 1e3 / steps) / 1e9
+    return {"mlups": LX * LY * steps / (ms / 1e3) / 1e6, "ms_per_step": ms / steps, "steps": steps,
+            "roofline_achieved_gbs": ach, "roofline_frac": ach / HBM_PEAK_GBS,
+            "workload": f"periodic {LX}x{LY}, {storage}, uniform flow, one step per launch"}
 
 
 def time_steps(lat, steps, warmup, torch):
@@ -527,6 +550,8 @@ def main():
                 "bytes_per_node_step": BYTES_PER_NODE[storage] // spl_, "traffic": measured_traffic(storage, spl_)}
 
     single = leg(args.storage, args.single_steps, 1) if spl == 2 else None
+    if single is not None and not args.no_extra:
+        single["periodic_lattice"] = periodic_single_step_leg(torch, args.storage)
     fp64 = leg("f64", args.fp64_steps, args.temporal) if args.storage != "f64" else None
     fp64_single = leg("f64", args.fp64_steps, 1) if args.storage != "f64" and args.temporal == 2 else None
     f16 = leg("f16", args.f16_steps, args.temporal) if args.storage != "f16" else None
