@@ -278,11 +278,19 @@ def _main(argv=None):
     ap.add_argument("--storage", default="f32")
     ap.add_argument("--sequential-jobs", type=int, default=0, help="solve only the first N one at a time (0: all)")
     ap.add_argument("--out", default=None)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batch with three and with two steps per launch (for kernel profiles)")
     a = ap.parse_args(argv)
     jobs = channel_jobs(a.jobs, seed=a.seed)
     res = {"jobs": a.jobs, "steps": a.steps, "seed": a.seed, "storage": a.storage,
            "nodes": sum(j.nodes for j in jobs), "nx_range": [min(j.lx for j in jobs), max(j.lx for j in jobs)]}
     print(json.dumps(res), flush=True)
+    if a.batched_only:
+        for temporal, name in ((3, "batched_three_step"), (2, "batched_two_step")):
+            sec, ml, _ = run_batched(jobs, a.steps, a.storage, temporal=temporal)
+            res[name] = {"seconds": sec, "mlups": ml}
+            print(json.dumps({name: res[name]}), flush=True)
+        return
     sec, ml, _ = run_batched(jobs, a.steps, a.storage, temporal=2)
     res["batched_two_step"] = {"seconds": sec, "mlups": ml,
                                "launch": "one launch per group of equal row width, groups on side streams"}
