@@ -1375,7 +1375,7 @@ __global__ __launch_bounds__(kPackedWaves * 64, 8) void k_sweep2_packed(const Sw
 // previous-stage values (its reflection sources) before that stage overwrites
 // them.  Stage s covers rows [xa - (K - s), xa + R + K - s), so the input is
 // read once plus K rows on each side, the output written once per K steps:
-// 36 / K B per node and step in FP32.  Every value goes through the same
+// 72 / K B per node and step in FP32 (24 B with K = 3).  Every value goes through the same
 // arithmetic as in single steps (identical results).  Needs R >= K - 1 rows in
 // every block; no pressure fix, at most kSweepMaxClasses classes.
 // ---------------------------------------------------------------------------
