@@ -27,8 +27,8 @@ def _channel(lx, ly, storage, temporal=1):
                   storage=storage)
     rho, u = O.density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), 0.1)
     lat.set_equilibrium(rho, u)
-    if temporal == 2:
-        lat.set_temporal(2)
+    if temporal > 1:
+        lat.set_temporal(temporal)
     return lat
 
 
@@ -66,7 +66,8 @@ def test_speed_snapshots_steps_and_files(tmp_path):
     lat.close()
 
 
-@pytest.mark.parametrize("storage,temporal", [("f64", 1), ("f32", 1), ("f32", 2), ("f16", 2), ("f64", 2)])
+@pytest.mark.parametrize("storage,temporal", [("f64", 1), ("f32", 1), ("f32", 2), ("f16", 2), ("f64", 2), ("f32", 3),
+                                              ("f16", 3)])
 def test_checkpoint_resume_is_bit_exact(storage, temporal, tmp_path):
     lx, ly = 52, 36
     om = 1 / 1.7

@@ -160,6 +160,30 @@ def test_k_steps_per_launch_equal_single_steps(lx, ly, storage, kind, steps, row
     assert np.array_equal(pa, pb)
 
 
+def test_k_steps_probe_ring_wraps_like_single_steps():
+    """A probe ring shorter than the run keeps the same last samples with
+    three steps per launch as with single steps (slots taken modulo the
+    capacity inside a launch)."""
+    lx, ly, steps = 30, 100, 23
+    probe = (lx - 3, 41)
+    from lattice_boltzmann_parallel_solver_amd import Lattice
+
+    outs = []
+    for k in (1, 3):
+        f, rho, u = _initial(lx, ly, 5)
+        lat = Lattice(lx, ly, boundary=_boundary("channel", lx, ly), storage="f32")
+        lat.set_state(f, rho, u)
+        lat.set_temporal(k)
+        lat.set_probe(probe[0], probe[1], 7)
+        lat.run(1.2, steps)
+        outs.append((lat.probe(), lat.state()))
+        lat.close()
+    assert outs[0][0].shape == (7, 3)
+    assert np.array_equal(outs[0][0], outs[1][0])
+    for x, y in zip(outs[0][1], outs[1][1]):
+        assert np.array_equal(x, y)
+
+
 def test_k_steps_rejected_where_unsupported():
     from lattice_boltzmann_parallel_solver_amd import Lattice
     from lattice_boltzmann_parallel_solver_amd._lib import LbmError
