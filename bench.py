@@ -541,7 +541,7 @@ def main():
         steps += steps % 2
         lat = build_lattice(storage, torch, temporal)
         spl_ = lat.steps_per_launch
-        ms_ = time_steps(lat, steps, max(6, args.warmup // 4), torch)
+        ms_ = time_steps(lat, steps, max(20, args.warmup), torch)
         lat.close()
         torch.cuda.empty_cache()
         ach = BYTES_PER_NODE[storage] * N / (ms_ / 1e3 / (steps / spl_)) / 1e9
