@@ -459,8 +459,8 @@ def f16_error_note():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "f64"])
     ap.add_argument("--single-steps", type=int, default=200, help="steps of the one-step-per-launch leg")
     ap.add_argument("--fp64-steps", type=int, default=200)
