@@ -59,7 +59,7 @@ hipError_t prepare_sweepk_batch(int storage, int K, size_t lds);
 bool sweep_batch_supported(int storage, const Geom &g, const BcView &bc);
 SweepShape sweep_shape(int storage, uint32_t ly, int tile_threads = 0, int tile_v = 0);
 // Output rows per block of the sweep kernel: cfg.sweep_rows if set, else
-// spread over whole waves of `slots` resident blocks (at least 8 rows).
+// spread over whole waves of `slots` resident blocks.
 int sweep_rows_for(const Geom &g, const LaunchCfg &cfg, int slots);
 // Nodes per thread of the 16-byte kernel for a storage type.
 int vec16_nodes(int storage);
