@@ -199,14 +199,18 @@ int lbm_batch_set_launch(lbm_batch *batch, int nodes_per_thread, int use_graph);
 int lbm_batch_run(lbm_batch *batch, int64_t nsteps);
 /* Two steps per launch for the batch (steps_per_launch = 2): every context
  * the two-step sweep kernel supports (no pressure fix, ny <= 2048 FP32/FP16 or
- * 1024 FP64) is advanced by it, grouped by block width, `rows_per_block` rows
- * per block (0 = 8; at least 32 for lattices wide enough for blocks of 512 or
- * more threads); the others keep the one-step batched kernel.
- * steps_per_launch = 3: as 2, and in batches of 8 Mi nodes or more the FP32
- * groups whose rows take 7-8 or 11-16 wavefronts (385-512 or 641-1024
- * columns) take three steps per launch, where that measured faster, each
- * group on a side stream.  Results are identical in every mode.  Default
- * 1.  No counterpart in the reference. */
+ * 1024 FP64) is advanced by it, grouped by block width, each group on a
+ * side stream when there are several, `rows_per_block` rows per block; the
+ * others keep the one-step batched kernel.  rows_per_block = 0 (automatic):
+ * 8, at least 32 for lattices wide enough for blocks of 512 or more threads,
+ * and in small batches fewer, so that the batch's rows still make about four
+ * blocks per CU (two for the wide lattices).
+ * steps_per_launch = 3: as 2, and the FP32 groups whose rows take 7-8 or
+ * 11-16 wavefronts (385-512 or 641-1024 columns) take three steps per launch,
+ * where that measured faster, on a side stream (a batch of one such group
+ * only from 8 Mi nodes; with automatic rows only in batches whose rows make 8
+ * per block at four blocks per CU).  Results are identical in every mode.
+ * Default 1.  No counterpart in the reference. */
 int lbm_batch_set_temporal(lbm_batch *batch, int steps_per_launch, int rows_per_block);
 /* With two steps per launch, FP32/FP16 contexts of at most 1024 columns can
  * be advanced by one packed launch per step pair: their row ranges,
@@ -215,7 +219,7 @@ int lbm_batch_set_temporal(lbm_batch *batch, int steps_per_launch, int rows_per_
  * captured in a HIP graph on the batch's one stream.  packed = 1: one block
  * per packed block; 2: a persistent grid of the blocks the GPU holds at once,
  * each sweeping every grid-th packed block; 0 (default): one kernel per group
- * of equal row width, on side streams in large batches (the fastest measured:
+ * of equal row width, on side streams (the fastest measured:
  * 130.2 GLUPS on 256 uneven channels against 126.7 packed and 120.3
  * persistent).  Results are identical in every mode.  No counterpart in the
  * reference. */

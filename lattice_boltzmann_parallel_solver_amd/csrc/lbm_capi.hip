@@ -770,6 +770,7 @@ struct lbm_batch {
     // two steps per launch for the jobs the sweep kernel supports
     int temporal = 1;
     int sweep_rows = 8;
+    bool rows_auto = true;  // rows per block not given: fewer rows for small batches
     // one packed launch for every job of one column per thread (FP32/FP16):
     // 0 off (per-width groups, the faster on MI355X), 1 one block per packed
     // block, 2 persistent blocks
@@ -856,9 +857,9 @@ bool three_steps_pay(int storage, unsigned waves) {
     return storage == LBM_STORE_F32 && (waves == 7 || waves == 8 || waves >= 11);
 }
 
-// Side streams of a batch, and the batch size (nodes) from which its groups
-// run on them: below it a pair of launches is short and the graph replay
-// on one stream costs less than the extra launches.
+// Side streams of a batch, and the batch size (nodes) from which a batch
+// whose jobs all fall into one three-step group runs it on a side stream
+// (several groups take side streams at any size).
 constexpr size_t kBatchLanes = 4;
 constexpr int64_t kBatchLaneNodes = 8 << 20;
 
@@ -883,10 +884,10 @@ hipError_t ensure_lanes(lbm_batch *b, size_t k) {
 
 // Steps of a batch with the two-step sweep kernel: jobs it supports are
 // grouped by block width (one launch per group per step pair), the others
-// take two one-step batched launches per pair.  A large batch of several
-// groups runs each group on a side stream; otherwise graph-captured pairs of
-// pairs replay the whole run on the batch stream.  An odd last step goes job
-// by job.
+// take two one-step batched launches per pair.  A batch of several groups
+// runs each group on a side stream; otherwise graph-captured pairs of pairs
+// replay the whole run on the batch stream.  An odd last step goes job by
+// job.
 int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     const int n = (int)b->ctxs.size();
     struct Bucket {
@@ -903,6 +904,27 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     std::vector<int> fjobs;  // one-step jobs
     std::vector<int> pjobs;  // jobs of the packed launch
     const bool can_pack = b->packed != 0 && b->storage != LBM_STORE_F64;
+    // A small batch has few rows to share out: with the row counts chosen
+    // for large batches its launches would hold fewer blocks than the GPU
+    // takes at once.  With automatic rows, no group gets more rows per block
+    // than leave, over the whole batch, about four blocks per CU for rows of
+    // fewer than 512 threads and two for wider rows (channels of 0.77 M nodes
+    // in all: 52.5 GLUPS at 8 rows, 86.6 at 2; of 2.1 M nodes with one
+    // 514-column channel: 64.4 GLUPS at one block per CU for its rows, 86.1
+    // at two; one 8192 x 2048 lattice: 131.7 and 130.1), and three steps per
+    // launch, whose halo rows weigh more in short blocks, wait for 8 such
+    // rows (at 2 rows: 68.2 GLUPS against 86.6 with two steps per launch).
+    int64_t rows_total = 0;
+    int cus = 0;
+    bool three_fill = true;
+    if (b->rows_auto) {
+        for (int j = 0; j < n; j++) rows_total += b->ctxs[j]->g.lx;
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        three_fill = rows_total / (4 * (int64_t)cus) >= 8;
+    }
     for (int j = 0; j < n; j++) {
         lbm_ctx *c = b->ctxs[j];
         if (!c->st.scratch || !sweep_batch_supported(c->storage, c->g, c->bc.view)) {
@@ -917,7 +939,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             pjobs.push_back(j);
             continue;
         }
-        const int k = b->temporal == 3 && v == 1 && !big && three_steps_pay(c->storage, threads / 64) &&
+        const int k = b->temporal == 3 && three_fill && v == 1 && !big && three_steps_pay(c->storage, threads / 64) &&
                               sweepk_supported(c->storage, c->g, c->bc.view, 3)
                           ? 3
                           : 2;
@@ -951,6 +973,10 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             // three-step groups at 8, 16, 24, 32 and 48 rows: 132, 143, 147,
             // 149 and 148 GLUPS)
             if (bk.k == 3) cfg.sweep_rows = std::max(cfg.sweep_rows, 4 * b->sweep_rows);
+            if (b->rows_auto) {
+                const int64_t slots = (int64_t)cus * (bk.threads >= 512 ? 2 : 4);
+                cfg.sweep_rows = (int)std::min<int64_t>(cfg.sweep_rows, std::max<int64_t>(1, rows_total / slots));
+            }
             uint32_t rows = (uint32_t)sweep_rows_for(c->g, cfg, 0);
             // the three-step kernel needs two rows in every block; the same
             // tables also serve its two-step launches
@@ -1128,12 +1154,16 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     // groups: the buckets, then the packed launch, then the one-step jobs
     const size_t gpacked = buckets.size(), gstep = buckets.size() + (npblocks ? 1 : 0);
     const size_t groups = gstep + (fjobs.empty() ? 0 : 1);
-    // three-step groups run only on side streams (their launch count differs
-    // from the pairs'), so a large batch holding one takes them even with a
-    // single group
+    // Several groups run on side streams at any batch size: graph replay of
+    // the pairs on one stream measured no faster than plain launches on this
+    // GPU, and one stream leaves each group's small launch alone on the GPU
+    // (4 uneven channels: 13.5 GLUPS on one stream, 52.5 on side streams; 16:
+    // 35.6 and 83.0).  Three-step groups run only on side streams (their
+    // launch count differs from the pairs'), so a large batch holding one
+    // takes them even with a single group.
     bool any_three = false;
     for (const Bucket &bk : buckets) any_three |= bk.k == 3;
-    if ((groups > 1 || any_three) && nodes >= (int64_t)kBatchLaneNodes && pairs >= 1) {
+    if ((groups > 1 || (any_three && nodes >= (int64_t)kBatchLaneNodes)) && pairs >= 1) {
         const size_t nl = std::min(groups, (size_t)kBatchLanes);
         LBM_HIP_CHECK(ensure_lanes(b, nl), "lbm_batch_run: side streams");
         // groups onto lanes, largest first onto the least loaded
@@ -1361,6 +1391,7 @@ int lbm_batch_set_temporal(lbm_batch *b, int steps_per_launch, int rows_per_bloc
     }
     b->temporal = steps_per_launch;
     b->sweep_rows = rows_per_block > 0 ? rows_per_block : 8;
+    b->rows_auto = rows_per_block <= 0;
     drop_graph(b);
     return LBM_OK;
 }

@@ -430,3 +430,61 @@ def test_packed_launch_equals_grouped_launches(storage, rows):
             want = N.run(O.equilibrium(rho, u), rho, u, 1.0 / (3 * 0.1 * d / 100.0 + 0.5), 26, bc)
             for a, w, name in zip(out[0][0][k], want[:3], ("f", "rho", "u")):
                 assert np.abs(a - w).max() <= F32_TOL[name], (k, name)
+
+
+SMALL_GROUPS = [(60, 480), (50, 700), (41, 200), (30, 96)]
+
+
+@pytest.mark.parametrize("temporal,shapes,rows", [
+    (2, [(40, 200), (33, 200), (57, 200)], 0),  # one group: graph-captured pairs, one row per block
+    (3, SMALL_GROUPS, 0),                       # several groups, too few rows for three steps per launch
+    (3, SMALL_GROUPS, 2),                       # the same with three steps for the 8- and 11-wave groups
+])
+def test_small_batch_rows_and_side_streams_equal_individual_runs(temporal, shapes, rows):
+    """Small batches: with automatic rows per block (few rows per block, so
+    that the batch's few rows still fill the GPU) one width group replays
+    captured step pairs on the batch stream and several groups run on side
+    streams with two steps per launch; with rows given, the FP32 groups of 8
+    and of 11 waves per row take three steps per launch on their side
+    streams.  States and probes equal each lattice's own single-step run, and
+    the FP32 oracle's after the last chunk."""
+    from lattice_boltzmann_parallel_solver_amd import Batch, Lattice
+    from lattice_boltzmann_parallel_solver_amd.boundary_utils import von_karman_boundary_conditions
+    from lattice_boltzmann_parallel_solver_amd.initial_values import density_1_velocity_x_u0_velocity_y_0_initial
+
+    chunks = (5, 8, 7, 4)
+
+    def build():
+        lats, oms = [], []
+        for lx, ly in shapes:
+            d = max(2, ly // 5)
+            lat = Lattice(lx, ly, boundary=von_karman_boundary_conditions(lx, ly, 1.0, 0.1, d, walls=True, y_offset=1),
+                          storage="f32")
+            rho, u = density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), 0.1)
+            lat.set_equilibrium(rho, u)
+            lat.set_probe(3 * lx // 4, ly // 2, 64)
+            lats.append(lat)
+            oms.append(1.0 / (3 * 0.1 * d / 100.0 + 0.5))
+        return lats, oms
+
+    lats, oms = build()
+    ref, _ = build()
+    b = Batch(lats, oms, temporal=temporal, rows_per_block=rows)
+    for chunk in chunks:
+        b.run(chunk)
+        for r, om in zip(ref, oms):
+            r.run(om, chunk)
+    b.close()
+    for lat, r in zip(lats, ref):
+        for a, c in zip(lat.state(), r.state()):
+            assert np.array_equal(a, c), (lat.lx, lat.ly)
+        assert np.array_equal(lat.probe(), r.probe()), (lat.lx, lat.ly)
+    total = sum(chunks)
+    want = []
+    for (lx, ly), om in zip(shapes, oms):
+        rho, u = O.density_1_velocity_x_u0_velocity_y_0_initial((lx, ly), 0.1)
+        bc = O.von_karman_boundary_conditions(lx, ly, 1.0, 0.1, max(2, ly // 5), walls=True, y_offset=1)
+        want.append(N.run(O.equilibrium(rho, u), rho, u, om, total, bc, probe=(3 * lx // 4, ly // 2)))
+    check_oracle(lats, want, "f32", probes=True)
+    for lat in lats + ref:
+        lat.close()
