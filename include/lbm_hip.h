@@ -204,7 +204,9 @@ int lbm_batch_run(lbm_batch *batch, int64_t nsteps);
  * others keep the one-step batched kernel.  rows_per_block = 0 (automatic):
  * 8, at least 32 for lattices wide enough for blocks of 512 or more threads,
  * and in small batches fewer, so that the batch's rows still make about four
- * blocks per CU (two for the wide lattices).
+ * blocks per CU (two for the wide lattices); where the wide lattices hold
+ * most of the batch, one row count for all of them that fills whole rounds
+ * of the blocks the GPU holds at once.
  * steps_per_launch = 3: as 2, and the FP32 groups whose rows take 7-8 or
  * 11-16 wavefronts (385-512 or 641-1024 columns) take three steps per launch,
  * where that measured faster, on a side stream (a batch of one such group

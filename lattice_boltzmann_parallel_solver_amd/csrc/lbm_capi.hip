@@ -20,6 +20,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "lbm_batch_tables.h"
@@ -911,9 +912,10 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     // fewer than 512 threads and two for wider rows (channels of 0.77 M nodes
     // in all: 52.5 GLUPS at 8 rows, 86.6 at 2; of 2.1 M nodes with one
     // 514-column channel: 64.4 GLUPS at one block per CU for its rows, 86.1
-    // at two; one 8192 x 2048 lattice: 131.7 and 130.1), and three steps per
-    // launch, whose halo rows weigh more in short blocks, wait for 8 such
-    // rows (at 2 rows: 68.2 GLUPS against 86.6 with two steps per launch).
+    // at two), unless the wide groups hold most of the batch (below).  Three
+    // steps per launch, whose halo rows weigh more in short blocks, wait for
+    // 8 such rows (at 2 rows: 68.2 GLUPS against 86.6 with two steps per
+    // launch).
     int64_t rows_total = 0;
     int cus = 0;
     bool three_fill = true;
@@ -956,6 +958,48 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
         }
         bk->members.push_back(j);
     }
+    // Wide rows (blocks of 512 threads or more) leave one to three blocks per
+    // CU, so their launches run in rounds of the blocks the GPU holds at
+    // once, and a last round that is barely filled costs as much as a full
+    // one.  With automatic rows the wide two-step groups share the row count
+    // that minimises rounds x (rows + 3) over all their blocks together (a
+    // block also reads and computes about three halo rows).  Measured, FP32
+    // GLUPS at 32 rows / at the chosen rows: four 4200 x 1800 channels 108.3 /
+    // 131.8 (33), one 2048 x 2048 lattice 50.7 / 115.2 (8), two 8192 x 2048
+    // channels with a 2048 and a 1500 square 124.0 / 133.1 (40), one 8192 x
+    // 2048 channel 133.0 (32 chosen; 34 gives 136.3).  Where the wide groups
+    // hold less than half of the swept nodes their blocks share the GPU with
+    // the narrow groups' and the rule for those applies to them too.
+    int wide_rows = 0;
+    int64_t wide_nodes = 0, swept_nodes = 0;
+    for (const Bucket &bk : buckets)
+        for (int j : bk.members) {
+            swept_nodes += b->ctxs[j]->g.N;
+            if (bk.k == 2 && bk.threads >= 512) wide_nodes += b->ctxs[j]->g.N;
+        }
+    if (b->rows_auto && 2 * wide_nodes >= swept_nodes) {
+        std::vector<std::pair<uint32_t, int>> wide;  // (lx, blocks per CU)
+        for (const Bucket &bk : buckets) {
+            if (bk.k != 2 || bk.threads < 512) continue;
+            size_t lds64 = 0;
+            for (int j : bk.members) {
+                const lbm_ctx *c = b->ctxs[j];
+                lds64 = std::max(lds64, sweep_lds_bytes(c->storage, c->g.ly, c->bc.view.chunk ? c->bc.view.nclasses : 0, 64));
+            }
+            const int per_cu = sweep2_batch_per_cu(b->storage, bk.v, bk.big, bk.threads, lds64);
+            for (int j : bk.members) wide.emplace_back(b->ctxs[j]->g.lx, per_cu);
+        }
+        double best = 0;
+        for (uint32_t r = 1; r <= 128 && !wide.empty(); r++) {
+            double slots = 0;  // CU share of the wide blocks at r rows per block
+            for (const auto &w : wide) slots += (double)((w.first + r - 1) / r) / w.second;
+            const double t = std::ceil(slots / cus - 1e-9) * (r + 3);
+            if (wide_rows == 0 || t < best) {
+                best = t;
+                wide_rows = (int)r;
+            }
+        }
+    }
     // tables: sweep jobs per bucket and pair parity with their block tables,
     // then the one-step jobs' records for the two steps of a pair
     std::vector<SweepGroupTables> gt(buckets.size());
@@ -974,8 +1018,11 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             // 149 and 148 GLUPS)
             if (bk.k == 3) cfg.sweep_rows = std::max(cfg.sweep_rows, 4 * b->sweep_rows);
             if (b->rows_auto) {
-                const int64_t slots = (int64_t)cus * (bk.threads >= 512 ? 2 : 4);
-                cfg.sweep_rows = (int)std::min<int64_t>(cfg.sweep_rows, std::max<int64_t>(1, rows_total / slots));
+                if (bk.k == 2 && bk.threads >= 512 && wide_rows > 0)
+                    cfg.sweep_rows = wide_rows;
+                else
+                    cfg.sweep_rows = (int)std::min<int64_t>(
+                        cfg.sweep_rows, std::max<int64_t>(1, rows_total / ((bk.threads >= 512 ? 2 : 4) * (int64_t)cus)));
             }
             uint32_t rows = (uint32_t)sweep_rows_for(c->g, cfg, 0);
             // the three-step kernel needs two rows in every block; the same

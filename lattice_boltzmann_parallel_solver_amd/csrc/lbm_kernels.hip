@@ -2440,6 +2440,16 @@ hipError_t prepare_sweep2_batch(int storage, int v, bool big, size_t lds) {
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
+int sweep2_batch_per_cu(int storage, int v, bool big, unsigned threads, size_t lds) {
+    const void *fn = sweep_batch_fn(storage, v, big);
+    int per_cu = 0;
+    if (!fn || threads == 0 || lds > kLdsLimit ||
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds) != hipSuccess)
+        return 1;
+    return per_cu > 0 ? per_cu : 1;
+}
+
 hipError_t launch_sweep2_batch(int storage, int v, bool big, const SweepJob *jobs, const uint32_t *block_job,
                                unsigned nblocks, unsigned threads, size_t lds, const int64_t *step, int64_t step_now,
                                hipStream_t s) {

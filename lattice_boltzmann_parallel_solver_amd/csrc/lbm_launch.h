@@ -81,6 +81,9 @@ int64_t batch_job_blocks(const Geom &g, int nodes_per_thread);
 hipError_t launch_sweep2_batch(int storage, int v, bool big, const SweepJob *jobs, const uint32_t *block_job,
                                unsigned nblocks, unsigned threads, size_t lds, const int64_t *step, int64_t step_now,
                                hipStream_t s);
+// Blocks of the batched sweep kernel of this shape one CU holds at once with
+// `lds` bytes of LDS each (1 if the runtime cannot say).
+int sweep2_batch_per_cu(int storage, int v, bool big, unsigned threads, size_t lds);
 // Allow `lds` bytes of dynamic LDS for the batched sweep kernel (call before
 // launching or capturing it).
 hipError_t prepare_sweep2_batch(int storage, int v, bool big, size_t lds);
