@@ -2443,8 +2443,10 @@ hipError_t prepare_sweep2_batch(int storage, int v, bool big, size_t lds) {
 int sweep2_batch_per_cu(int storage, int v, bool big, unsigned threads, size_t lds) {
     const void *fn = sweep_batch_fn(storage, v, big);
     int per_cu = 0;
+    // the limit raised to the most LDS a block may take, never lowered: a
+    // batch launching this kernel meanwhile may need more than `lds`
     if (!fn || threads == 0 || lds > kLdsLimit ||
-        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds) != hipSuccess)
         return 1;
     return per_cu > 0 ? per_cu : 1;
