@@ -21,8 +21,11 @@ roofline   : HBM bound; achieved = algorithmic bytes per launch (every populatio
              1 GiB streaming copy (16 bytes per lane) timed the same way, the
              practical ceiling, and frac_of_copy = achieved / copy_gbs.
              row_sweep_pattern_gbs: the two-step kernel's access pattern with no
-             work (lbm_row_sweep_probe), the ceiling of a row sweep on this
-             layout, and frac_of_row_sweep_pattern = achieved / that.
+             work (lbm_row_sweep_probe_ordered) at the headline's rows per
+             block, in the range-interleaved row order its long runs use: the
+             ceiling of a row sweep on this layout, and
+             frac_of_row_sweep_pattern = achieved / that;
+             row_sweep_pattern_plain_order_gbs: the same in the plain order.
 single_step: the same workload with one step per launch (k_fused); its
              periodic_lattice entry times the same kernel on a fully periodic
              8192x2048 lattice (no boundary nodes).
@@ -144,11 +147,13 @@ def copy_gbs(torch, nbytes=1 << 30, reps=10):
     return 2 * nbytes * reps / (ms / 1e3) / 1e9
 
 
-def row_sweep_gbs(torch, rows=32, reps=10):
+def row_sweep_gbs(torch, rows=32, reps=10, interleaved=True):
     """HBM bandwidth of the two-step kernel's access pattern with no work
-    (the bench lattice's shape: nine planes of 2048 FP32 values per row, 32
-    rows per block, one row prefetched, a barrier per row), read + write
-    counted: the ceiling a row sweep reaches on this layout."""
+    (the bench lattice's shape: nine planes of 2048 FP32 values per row,
+    `rows` rows per block, one row prefetched, a barrier per row), read +
+    write counted, with the rows in the range-interleaved order of long
+    two-step runs or in the plain order: the ceiling a row sweep reaches on
+    this layout."""
     import ctypes
 
     from lattice_boltzmann_parallel_solver_amd import _lib
@@ -160,8 +165,8 @@ def row_sweep_gbs(torch, rows=32, reps=10):
     L = _lib.load()
 
     def once():
-        _lib.check(L.lbm_row_sweep_probe(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), LX, pitch,
-                                         rows, ctypes.c_void_p(stream)))
+        _lib.check(L.lbm_row_sweep_probe_ordered(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), LX,
+                                                 pitch, rows, 1 if interleaved else 0, ctypes.c_void_p(stream)))
 
     once()
     torch.cuda.synchronize()
@@ -515,7 +520,8 @@ def main():
         # block timed on a scratch lattice of the same shape), outside the timed region
         from lattice_boltzmann_parallel_solver_amd import workloads as W
 
-        best = W.autotune_lattice(lat, omega(), steps=60)
+        # (trials of 200 steps: long enough for the interleaved row order)
+        best = W.autotune_lattice(lat, omega(), steps=200)
         lat.steps_per_launch = best[3]
         tuned = {"nodes_per_thread": best[0], "xcd_order": best[1], "nt_stores": best[2],
                  "steps_per_launch": best[3], "rows_per_block": best[4]}
@@ -559,7 +565,11 @@ def main():
         f16["error_vs_fp64"] = f16_error_note()
 
     copy = copy_gbs(torch)
-    sweep_pattern = row_sweep_gbs(torch)
+    # the headline's rows per block (automatic: 32 on 256 CUs), in the row
+    # order its long two-step run uses and in the plain one
+    sweep_rows = (tuned or {}).get("rows_per_block") or 32
+    sweep_pattern = row_sweep_gbs(torch, rows=sweep_rows, interleaved=True)
+    sweep_pattern_plain = row_sweep_gbs(torch, rows=sweep_rows, interleaved=False)
     three = None if args.no_extra else three_step_leg(torch)
     configs = None if args.no_extra else config_legs(torch)
     latency = None if args.no_serving else serving_leg()
@@ -590,7 +600,8 @@ def main():
                      "bytes_per_node_step": BYTES_PER_NODE[args.storage] // spl,
                      "single_step_equivalent_gbs": mlups * 1e6 * BYTES_PER_NODE[args.storage] / 1e9,
                      "copy_gbs": copy, "frac_of_copy": achieved / copy,
-                     "row_sweep_pattern_gbs": sweep_pattern, "frac_of_row_sweep_pattern": achieved / sweep_pattern},
+                     "row_sweep_pattern_gbs": sweep_pattern, "frac_of_row_sweep_pattern": achieved / sweep_pattern,
+                     "row_sweep_pattern_plain_order_gbs": sweep_pattern_plain},
         "single_step": single,
         "three_steps_per_launch": three,
         "other_workloads_fp64": configs,

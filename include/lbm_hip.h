@@ -180,6 +180,17 @@ int lbm_set_temporal(lbm_ctx *ctx, int steps_per_launch, int rows_per_block);
  * other values, or when the tiles would be empty.  No counterpart in the
  * reference (a launch setting). */
 int lbm_set_sweep_tile(lbm_ctx *ctx, int threads, int cols_per_thread);
+/* Row order of long two-step runs.  interleave = 1 (default): an lbm_run of
+ * at least 64 steps with two steps per launch first copies the state into an
+ * order that stores offset t of every row range of the sweep next to each
+ * other (so the rows that all blocks read and write at once are neighbours in
+ * memory), sweeps there, and copies it back before returning -- for FP32 rows
+ * of two columns per thread (ny > 1024) and FP64 column tiles (ny > 1024)
+ * with 16 or more rows per block, where it measured faster; 0: the plain row
+ * order throughout.  Results are identical; the state is in the plain order
+ * whenever lbm_run has returned.  LBM_ERR_ARG for other values.  No
+ * counterpart in the reference (a layout setting of this library). */
+int lbm_set_row_layout(lbm_ctx *ctx, int interleave);
 
 /* ---- batches: many lattices advanced by one launch per step ------------- */
 /* A batch groups contexts (same device, same storage) so that one kernel
@@ -240,6 +251,13 @@ int lbm_copy_probe(const void *src, void *dst, int64_t bytes, void *stream);
  * on `stream`.  The bandwidth ceiling of a row sweep, beside the copy's.
  * pitch even, at most 4096.  No counterpart in the reference. */
 int lbm_row_sweep_probe(const void *src, void *dst, int nx, int pitch, int rows, void *stream);
+/* lbm_row_sweep_probe with the rows in the plain order (interleaved = 0) or
+ * in the range-interleaved order of long two-step runs (1: offset t of every
+ * `rows`-row range stored together, lbm_set_row_layout; rows <= 128, nx <
+ * 2^24).  Every value of src is transformed into the same place of dst in
+ * either order.  No counterpart in the reference. */
+int lbm_row_sweep_probe_ordered(const void *src, void *dst, int nx, int pitch, int rows, int interleaved,
+                                void *stream);
 
 /* ---- reductions that allocate nothing ----------------------------------- */
 /* Largest ncells lbm_moments_sum_noalloc accepts (174762 blocks of 256). */

@@ -96,6 +96,7 @@ _SIGNATURES = {
     "lbm_set_launch": (_I, [_P, _I, _I, _I]),
     "lbm_set_temporal": (_I, [_P, _I, _I]),
     "lbm_set_sweep_tile": (_I, [_P, _I, _I]),
+    "lbm_set_row_layout": (_I, [_P, _I]),
     "lbm_batch_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double), _I]),
     "lbm_batch_destroy": (_I, [_P]),
     "lbm_batch_set_launch": (_I, [_P, _I, _I]),
@@ -112,6 +113,7 @@ _SIGNATURES = {
     "lbm_moments_sum_noalloc": (_I, [_P, _I64, _P, _P]),
     "lbm_copy_probe": (_I, [_P, _P, _I64, _P]),
     "lbm_row_sweep_probe": (_I, [_P, _P, _I, _I, _I, _P]),
+    "lbm_row_sweep_probe_ordered": (_I, [_P, _P, _I, _I, _I, _I, _P]),
 }
 
 _lib = None

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "lbm_batch_tables.h"
+#include "lbm_rowlayout.h"
 #include "lbm_launch.h"
 
 namespace lbm {
@@ -127,6 +128,11 @@ struct lbm_ctx {
 };
 
 namespace {
+
+// Two-step runs of at least this many launches take the range-interleaved row
+// order (moving the state there and back costs about two launches).
+constexpr int64_t kInterleaveMinPairs = 32;
+constexpr int kInterleaveMinRows = 16;
 
 // Bytes of one state buffer: nine planes of lx rows, `pitch` elements a row.
 size_t plane_bytes(const lbm_ctx *c) { return 9 * (size_t)c->g.lx * c->g.pitch * plane_elem(c->storage); }
@@ -335,6 +341,17 @@ int lbm_set_sweep_tile(lbm_ctx *c, int threads, int cols_per_thread) {
     return LBM_OK;
 }
 
+int lbm_set_row_layout(lbm_ctx *c, int interleave) {
+    int rc = check_ctx(c, "lbm_set_row_layout");
+    if (rc) return rc;
+    if (interleave != 0 && interleave != 1) {
+        set_error("lbm_set_row_layout: interleave must be 0 or 1");
+        return LBM_ERR_ARG;
+    }
+    c->cfg.row_interleave = interleave;
+    return LBM_OK;
+}
+
 // Materialise f_n, rho_n, u_n of a running context into (device) outputs.
 static int finalize_into(lbm_ctx *c, double *f, double *rho, double *u) {
     LBM_HIP_CHECK(launch_finalize(c->storage, c->g, c->st, c->cur, c->bc.view, f, rho, u, nullptr, c->stream,
@@ -442,14 +459,44 @@ int lbm_run(lbm_ctx *c, double omega, int64_t nsteps) {
         }
     }
     if (K >= 2 && sweep_supported(c->storage, c->g, c->bc.view, c->cfg.sweep_tile, c->cfg.sweep_v)) {
+        // A long run moves the state into the range-interleaved row order of
+        // the sweep's row ranges (the rows all blocks touch at once become
+        // neighbours in memory) and back at its end: two copies of the state.
+        // Measured where it pays: FP32 rows of two columns per thread and FP64
+        // column tiles, with 16 or more rows per block (8192 x 2048 FP32
+        // 137.3 -> 147.4 GLUPS, FP64 67.7 -> 69.4, 4200 x 1800 FP32 125.5 ->
+        // 128.3).  It lost on one column per thread (8192 x 1024 FP32 130.1 ->
+        // 121.5), at 8 rows per block (2048 x 2048 FP32 125.7 -> 122.9) and in
+        // FP16 (220.5 -> 214.4), which keep the plain order.
+        LaunchCfg cfg = c->cfg;
+        bool ilv = false;
+        const SweepShape sh = sweep_shape(c->storage, c->g.ly, cfg.sweep_tile, cfg.sweep_v);
+        if (cfg.row_interleave && left / 2 >= kInterleaveMinPairs && c->g.lx < kRowLayoutMaxLx &&
+            c->storage != LBM_STORE_F16 && (sh.v == 2 || sh.ntiles > 1)) {
+            const int rows = sweep2_rows(c->storage, c->g, c->bc.view, cfg);
+            if (rows >= kInterleaveMinRows && rows <= (int)kRowLayoutMaxRows && rows < (int)c->g.lx) {
+                LBM_HIP_CHECK(launch_row_layout(c->storage, c->g, c->st, c->cur, (uint32_t)rows, true, c->stream),
+                              "row order kernel");
+                c->cur ^= 1;
+                cfg.sweep_rows = rows;
+                cfg.sweep_il = 1;
+                ilv = true;
+            }
+        }
         for (; left >= 2; left -= 2) {
             // materialises f_{steps} and f_{steps+1}
             const int64_t slot = probing ? c->steps - c->probe_base - 1 : -1;
             LBM_HIP_CHECK(launch_sweep2(c->storage, c->g, c->st, c->cur, c->bc.view, omega, c->probe, slot,
-                                        probing ? slot + 1 : -1, c->cfg, c->stream),
+                                        probing ? slot + 1 : -1, cfg, c->stream),
                           "two-step sweep kernel");
             c->cur ^= 1;
             c->steps += 2;
+        }
+        if (ilv) {
+            LBM_HIP_CHECK(launch_row_layout(c->storage, c->g, c->st, c->cur, (uint32_t)cfg.sweep_rows, false,
+                                            c->stream),
+                          "row order kernel");
+            c->cur ^= 1;
         }
     }
     for (; left > 0; left--) {
@@ -1613,15 +1660,26 @@ int lbm_copy_probe(const void *src, void *dst, int64_t bytes, void *stream) {
     return LBM_OK;
 }
 
-int lbm_row_sweep_probe(const void *src, void *dst, int nx, int pitch, int rows, void *stream) {
-    if (nx <= 0 || rows <= 0 || pitch <= 0 || pitch % 2 != 0 || pitch > 4096 || !src || !dst) {
-        set_error("lbm_row_sweep_probe: need device buffers, nx > 0, rows > 0 and an even pitch up to 4096");
+int lbm_row_sweep_probe_ordered(const void *src, void *dst, int nx, int pitch, int rows, int interleaved,
+                                void *stream) {
+    if (nx <= 0 || rows <= 0 || pitch <= 0 || pitch % 2 != 0 || pitch > 4096 || !src || !dst ||
+        (interleaved != 0 && interleaved != 1)) {
+        set_error("lbm_row_sweep_probe: need device buffers, nx > 0, rows > 0, an even pitch up to 4096 and "
+                  "interleaved 0 or 1");
         return LBM_ERR_ARG;
     }
-    LBM_HIP_CHECK(launch_row_sweep_probe(src, dst, (uint32_t)nx, (uint32_t)pitch, (uint32_t)rows,
+    if (interleaved && (rows > (int)kRowLayoutMaxRows || (uint32_t)nx >= kRowLayoutMaxLx)) {
+        set_error("lbm_row_sweep_probe: the interleaved order needs rows <= 128 and nx < 2^24");
+        return LBM_ERR_ARG;
+    }
+    LBM_HIP_CHECK(launch_row_sweep_probe(src, dst, (uint32_t)nx, (uint32_t)pitch, (uint32_t)rows, interleaved != 0,
                                          static_cast<hipStream_t>(stream)),
                   "lbm_row_sweep_probe");
     return LBM_OK;
+}
+
+int lbm_row_sweep_probe(const void *src, void *dst, int nx, int pitch, int rows, void *stream) {
+    return lbm_row_sweep_probe_ordered(src, dst, nx, pitch, rows, 0, stream);
 }
 
 int lbm_batch_run(lbm_batch *b, int64_t nsteps) {

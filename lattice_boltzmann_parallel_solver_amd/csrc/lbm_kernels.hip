@@ -27,6 +27,7 @@
 #include <type_traits>
 
 #include "lbm_launch.h"
+#include "lbm_rowlayout.h"
 
 namespace lbm {
 
@@ -799,6 +800,7 @@ struct SweepParams {
     int nt;
     int xcd;         // XCD-aware block order (neighbouring row ranges share an L2)
     S *scratch;      // 9 V blockDim.x values, written and never read
+    RowLayout il;    // row order of src and dst (rows = 0: plain)
 };
 
 template <class S>
@@ -1006,16 +1008,18 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
         return gcodes[row_at(rr) * ly + ys[k]];
     };
     // the pulls of row iteration rr for the lane's columns k0 <= k < k1
+    // (rows are addressed through their slots in the buffers' row order)
     auto pull_row = [&](uint32_t rr, const uint32_t code[V], C out[V][9], auto k0_tag, auto k1_tag) {
         constexpr int k0 = decltype(k0_tag)::value, k1 = decltype(k1_tag)::value;
         const uint32_t xr = row_at(rr), xm = xr == 0 ? lx - 1 : xr - 1, xp = xr + 1 == lx ? 0 : xr + 1;
+        const uint32_t sr = row_slot(p.il, xr), sm = row_slot(p.il, xm), sp = row_slot(p.il, xp);
 #pragma unroll
         for (int k = k0; k < k1; k++) {
             if (special(rr, k)) {
                 const uint32_t w = code[k] ? cls[code[k]].src : 0u;
-                pull32_bc<S, kBig>(gsrc, pitch, xm, xr, xp, ys[k], ym[k], yp[k], w, out[k]);
+                pull32_bc<S, kBig>(gsrc, pitch, sm, sr, sp, ys[k], ym[k], yp[k], w, out[k]);
             } else {
-                pull32<S, kBig>(gsrc, pitch, xm, xr, xp, ys[k], ym[k], yp[k], out[k]);
+                pull32<S, kBig>(gsrc, pitch, sm, sr, sp, ys[k], ym[k], yp[k], out[k]);
             }
         }
     };
@@ -1136,8 +1140,9 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
             Y[7] = lds_get<S>(exb + 4 * cs + lplu(k));
             Y[8] = lds_get<S>(exb + 5 * cs + lplu(k));
             if constexpr (kS2) {
-                // step 2 on row r = x - 1
+                // step 2 on row r = x - 1 (stored at slot sr2)
                 const uint32_t r = row_at(it - 1);
+                const uint32_t sr2 = row_slot(p.il, r);
                 C v2[9];
                 v2[0] = B[k][0];
                 v2[2] = B[k][1];
@@ -1184,7 +1189,7 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
                     const uint32_t lu = tid + k * T;
                     const uint32_t vc = cy0 + lu - 1;  // lu = 0 only with the left halo
                     const bool keep = lu >= 1 && (lu <= W || (vc >= ly && vc < pitch));
-                    GS *base = keep ? gdst + (size_t)r * 9 * pitch + vc : gscr + (size_t)k * 9 * T + tid;
+                    GS *base = keep ? gdst + (size_t)sr2 * 9 * pitch + vc : gscr + (size_t)k * 9 * T + tid;
                     const uint32_t stride = keep ? pitch : T;
 #pragma unroll
                     for (int q = 0; q < 9; q++) Io<S>::st(base + (size_t)q * stride, o2.s[q], p.nt != 0);
@@ -1192,7 +1197,7 @@ __device__ __forceinline__ void sweep_block(const SweepParams<S> &p, uint32_t xa
                     const uint32_t yst = min(tid + k * T, pitch - 1);
 #pragma unroll
                     for (int q = 0; q < 9; q++)
-                        Io<S>::st(pop32<kBig>(gdst, pitch, r, q, yst), o2.s[q], p.nt != 0);
+                        Io<S>::st(pop32<kBig>(gdst, pitch, sr2, q, yst), o2.s[q], p.nt != 0);
                 }
                 if ((flags2 & kFlagOutletSide) && owned(k)) {
                     gside[(size_t)y * 3 + 0] = v2[3];
@@ -2154,6 +2159,29 @@ const void *sweep_fn(bool probing, bool tiled, bool big) {
     return probing ? sweep_fn_p<S, VV, true>(tiled, big) : sweep_fn_p<S, VV, false>(tiled, big);
 }
 
+// Output rows per block of the two-step launch for this lattice and shape.
+template <class S>
+uint32_t sweep_rows_typed(const Geom &g, const BcView &bc, const LaunchCfg &cfg) {
+    const SweepShape sh = sweep_shape(sizeof(S) == 8 ? LBM_STORE_F64 : (sizeof(S) == 4 ? LBM_STORE_F32 : LBM_STORE_F16),
+                                      g.ly, cfg.sweep_tile, cfg.sweep_v);
+    const bool big = sweep_big(sizeof(S) == 8 ? LBM_STORE_F64 : (sizeof(S) == 4 ? LBM_STORE_F32 : LBM_STORE_F16), g);
+    constexpr int V2 = sizeof(S) == 8 ? 1 : 2;
+    int slots = 0;
+    if (cfg.sweep_rows <= 0) {
+        // resident blocks per CU (LDS and registers) times the CU count, in
+        // row ranges (the tiles of a range are separate blocks)
+        int dev = 0, cus = 0, per_cu = 0;
+        const size_t lds_est = sweep_lds_layout<S>(sh.lw, bc.chunk ? bc.nclasses : 0, 64).total;
+        const void *fn = sh.v == 1 ? sweep_fn<S, 1>(false, sh.ntiles > 1, big) : sweep_fn<S, V2>(false, sh.ntiles > 1, big);
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_est) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)sh.threads, lds_est) == hipSuccess)
+            slots = std::max(1, cus * (per_cu > 0 ? per_cu : 1) / (int)sh.ntiles);
+    }
+    return (uint32_t)sweep_rows_for(g, cfg, slots);
+}
+
 template <class S>
 hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView &bc, double omega,
                        const ProbeView &pr, int64_t slot1, int64_t slot2, const LaunchCfg &cfg, hipStream_t s) {
@@ -2187,20 +2215,12 @@ hipError_t sweep_typed(const Geom &g, const StateView &st, int src, const BcView
     constexpr int V2 = sizeof(S) == 8 ? 1 : 2;  // FP64 rows are one column per thread
     if (V != 1 && V != V2) return hipErrorInvalidValue;
     auto pick = [&](bool probe) { return V == 1 ? sweep_fn<S, 1>(probe, tiled, big) : sweep_fn<S, V2>(probe, tiled, big); };
-    int slots = 0;
-    if (cfg.sweep_rows <= 0) {
-        // resident blocks per CU (LDS and registers) times the CU count, in
-        // row ranges (the tiles of a range are separate blocks)
-        int dev = 0, cus = 0, per_cu = 0;
-        const size_t lds_est = sweep_lds_layout<S>(sh.lw, bc.chunk ? bc.nclasses : 0, 64).total;
-        const void *fn = pick(false);
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_est) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds_est) == hipSuccess)
-            slots = std::max(1, cus * (per_cu > 0 ? per_cu : 1) / (int)sh.ntiles);
+    p.rows = sweep_rows_typed<S>(g, bc, cfg);
+    // the buffers in the range-interleaved order of exactly these row ranges
+    if (cfg.sweep_il) {
+        p.il = row_layout(g.lx, p.rows);
+        if (p.il.rows != p.rows) return hipErrorInvalidValue;
     }
-    p.rows = (uint32_t)sweep_rows_for(g, cfg, slots);
     const unsigned grid = (g.lx + p.rows - 1) / p.rows * sh.ntiles;
     const size_t lds = sweep_lds_layout<S>(sh.lw, bc.chunk ? bc.nclasses : 0, p.rows).total;
     const void *fn = pick(probing);
@@ -2285,6 +2305,57 @@ hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int sr
         case LBM_STORE_F16: return sweep_typed<__half>(g, st, src, bc, omega, pr, slot1, slot2, cfg, s);
     }
     return hipErrorInvalidValue;
+}
+
+int sweep2_rows(int storage, const Geom &g, const BcView &bc, const LaunchCfg &cfg) {
+    switch (storage) {
+        case LBM_STORE_F64: return (int)sweep_rows_typed<double>(g, bc, cfg);
+        case LBM_STORE_F32: return (int)sweep_rows_typed<float>(g, bc, cfg);
+        case LBM_STORE_F16: return (int)sweep_rows_typed<__half>(g, bc, cfg);
+    }
+    return 0;
+}
+
+namespace {
+// Row x of src to its slot in dst (kTo) or back (!kTo): one block per row,
+// 16-byte copies of its nine planes (9 pitch values, a multiple of 128
+// bytes).  Block 0 also copies the outlet buffer (n_side 4-byte words).
+template <bool kTo>
+__global__ __launch_bounds__(kBlock) void k_row_layout(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                       uint32_t row16, RowLayout L, const uint32_t *__restrict__ side_src,
+                                                       uint32_t *__restrict__ side_dst, uint32_t n_side) {
+    const uint32_t x = blockIdx.x;
+    const uint32_t sl = row_slot(L, x);
+    const uint4 *in = src + (size_t)(kTo ? x : sl) * row16;
+    uint4 *out = dst + (size_t)(kTo ? sl : x) * row16;
+    for (uint32_t i = threadIdx.x; i < row16; i += kBlock) out[i] = in[i];
+    if (x == 0)
+        for (uint32_t i = threadIdx.x; i < n_side; i += kBlock) side_dst[i] = side_src[i];
+}
+}  // namespace
+
+hipError_t launch_row_layout(int storage, const Geom &g, const StateView &st, int src, uint32_t rows, bool to_ilv,
+                             hipStream_t s) {
+    const RowLayout L = row_layout(g.lx, rows);
+    if (L.rows != rows || rows == 0) return hipErrorInvalidValue;
+    const size_t elem = storage == LBM_STORE_F64 ? 8 : (storage == LBM_STORE_F32 ? 4 : 2);
+    const size_t row_bytes = 9 * (size_t)g.pitch * elem;
+    if (row_bytes % 16 != 0 || row_bytes / 16 > 0xffffffffull) return hipErrorInvalidValue;
+    // outlet buffers: 3 ly values of the compute type (FP64 or FP32)
+    const size_t side_bytes = 3 * (size_t)g.ly * (storage == LBM_STORE_F64 ? 8 : 4);
+    const void *in = st.plane[src];
+    void *out = st.plane[src ^ 1];
+    if (to_ilv)
+        hipLaunchKernelGGL(k_row_layout<true>, dim3(g.lx), dim3(kBlock), 0, s, static_cast<const uint4 *>(in),
+                           static_cast<uint4 *>(out), (uint32_t)(row_bytes / 16), L,
+                           static_cast<const uint32_t *>(st.side[src]), static_cast<uint32_t *>(st.side[src ^ 1]),
+                           (uint32_t)(side_bytes / 4));
+    else
+        hipLaunchKernelGGL(k_row_layout<false>, dim3(g.lx), dim3(kBlock), 0, s, static_cast<const uint4 *>(in),
+                           static_cast<uint4 *>(out), (uint32_t)(row_bytes / 16), L,
+                           static_cast<const uint32_t *>(st.side[src]), static_cast<uint32_t *>(st.side[src ^ 1]),
+                           (uint32_t)(side_bytes / 4));
+    return hipGetLastError();
 }
 
 namespace {
@@ -2537,7 +2608,7 @@ __global__ __launch_bounds__(kBlock) void k_copy16(const uint4 *__restrict__ src
 // lane (pitch / 2 <= NS blockDim.x).
 template <int NS>
 __global__ __launch_bounds__(1024) void k_row_sweep_probe(const float2 *__restrict__ src, float2 *__restrict__ dst,
-                                                          uint32_t lx, uint32_t pitch2, uint32_t rows) {
+                                                          uint32_t lx, uint32_t pitch2, uint32_t rows, RowLayout L) {
     const uint32_t xa = xcd_block(blockIdx.x, gridDim.x) * rows;
     const uint32_t xe = min(xa + rows, lx);
     const uint32_t T = blockDim.x;
@@ -2545,24 +2616,27 @@ __global__ __launch_bounds__(1024) void k_row_sweep_probe(const float2 *__restri
 #pragma unroll
     for (int k = 0; k < NS; k++) col[k] = min(threadIdx.x + k * T, pitch2 - 1);
     float2 cur[NS][9], nxt[NS][9];
+    const size_t s0 = row_slot(L, xa);
 #pragma unroll
     for (int k = 0; k < NS; k++)
 #pragma unroll
-        for (int i = 0; i < 9; i++) cur[k][i] = src[((size_t)xa * 9 + i) * pitch2 + col[k]];
+        for (int i = 0; i < 9; i++) cur[k][i] = src[(s0 * 9 + i) * pitch2 + col[k]];
     for (uint32_t x = xa; x < xe; x++) {
         if (x + 1 < xe) {
+            const size_t s1 = row_slot(L, x + 1);
 #pragma unroll
             for (int k = 0; k < NS; k++)
 #pragma unroll
-                for (int i = 0; i < 9; i++) nxt[k][i] = src[((size_t)(x + 1) * 9 + i) * pitch2 + col[k]];
+                for (int i = 0; i < 9; i++) nxt[k][i] = src[(s1 * 9 + i) * pitch2 + col[k]];
         }
         // lanes past the last slot write the last one again (same values)
+        const size_t sx = row_slot(L, x);
 #pragma unroll
         for (int k = 0; k < NS; k++)
 #pragma unroll
             for (int i = 0; i < 9; i++)
-                dst[((size_t)x * 9 + i) * pitch2 + col[k]] = make_float2(fmaf(cur[k][i].x, 1.0001f, 0.5f),
-                                                                         fmaf(cur[k][i].y, 0.9999f, 0.25f));
+                dst[(sx * 9 + i) * pitch2 + col[k]] = make_float2(fmaf(cur[k][i].x, 1.0001f, 0.5f),
+                                                                  fmaf(cur[k][i].y, 0.9999f, 0.25f));
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < NS; k++)
@@ -2572,17 +2646,22 @@ __global__ __launch_bounds__(1024) void k_row_sweep_probe(const float2 *__restri
 }
 
 hipError_t launch_row_sweep_probe(const void *src, void *dst, uint32_t lx, uint32_t pitch, uint32_t rows,
-                                  hipStream_t s) {
+                                  bool interleaved, hipStream_t s) {
     if (lx == 0 || rows == 0 || pitch == 0 || pitch % 2 != 0 || pitch > 4096) return hipErrorInvalidValue;
+    RowLayout L;
+    if (interleaved) {
+        L = row_layout(lx, rows);
+        if (L.rows != rows) return hipErrorInvalidValue;
+    }
     const uint32_t pitch2 = pitch / 2;
     const unsigned grid = (lx + rows - 1) / rows;
     if (pitch2 <= 1024) {
         const uint32_t threads = (pitch2 + 63) / 64 * 64;
         hipLaunchKernelGGL(k_row_sweep_probe<1>, dim3(grid), dim3(threads), 0, s, static_cast<const float2 *>(src),
-                           static_cast<float2 *>(dst), lx, pitch2, rows);
+                           static_cast<float2 *>(dst), lx, pitch2, rows, L);
     } else {
         hipLaunchKernelGGL(k_row_sweep_probe<2>, dim3(grid), dim3(1024), 0, s, static_cast<const float2 *>(src),
-                           static_cast<float2 *>(dst), lx, pitch2, rows);
+                           static_cast<float2 *>(dst), lx, pitch2, rows, L);
     }
     return hipGetLastError();
 }

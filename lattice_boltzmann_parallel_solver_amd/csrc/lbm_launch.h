@@ -28,6 +28,14 @@ int fused_blocks(const Geom &g, const LaunchCfg &cfg);
 hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int src, const BcView &bc,
                          double omega, const ProbeView &pr, int64_t probe_slot1, int64_t probe_slot2,
                          const LaunchCfg &cfg, hipStream_t s);
+// Output rows per block that launch_sweep2 uses with this configuration.
+int sweep2_rows(int storage, const Geom &g, const BcView &bc, const LaunchCfg &cfg);
+// planes[src] -> planes[src^1] and side[src] -> side[src^1], rows moved to
+// (to_ilv) or from their slots in the range-interleaved order of `rows`-row
+// ranges (lbm_rowlayout.h); launch_sweep2 with cfg.sweep_il and cfg.sweep_rows
+// = rows then runs on buffers in that order.
+hipError_t launch_row_layout(int storage, const Geom &g, const StateView &st, int src, uint32_t rows, bool to_ilv,
+                             hipStream_t s);
 // The sweep kernel needs no pressure fix, at most 128 boundary classes,
 // lx >= 3, the state buffer (9 lx pitch values) addressable with 32-bit byte
 // offsets, and its LDS (12 (cols + 1) storage values, 6 cols compute values,
@@ -108,9 +116,10 @@ bool sweep_big(int storage, const Geom &g);
 // LDS bytes of the sweep kernel for a lattice of this ly, class count and rows per block.
 size_t sweep_lds_bytes(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows);
 // The two-step kernel's row-sweep access pattern with no work (measurement
-// only): FP32 buffers of 9 lx pitch values, `rows` rows per block.
+// only): FP32 buffers of 9 lx pitch values, `rows` rows per block, rows in
+// the plain or the range-interleaved order of those ranges.
 hipError_t launch_row_sweep_probe(const void *src, void *dst, uint32_t lx, uint32_t pitch, uint32_t rows,
-                                  hipStream_t s);
+                                  bool interleaved, hipStream_t s);
 // dst[i] = src[i] for n16 16-byte chunks (streaming copy, measurement only).
 hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t s);
 // *step += k (one thread).

@@ -65,6 +65,8 @@ struct LaunchCfg {
     int sweep_rows = 0;        // output rows per block of the sweep kernel (0: automatic)
     int sweep_tile = 0;        // threads per column tile of the sweep kernel (0: whole rows up to one block)
     int sweep_v = 0;           // columns per thread of column tiles (0: two for FP32/FP16, one for FP64)
+    int row_interleave = 1;    // long two-step runs move the state to the range-interleaved row order
+    int sweep_il = 0;          // (per launch) the buffers are in that order for sweep_rows-row ranges
 };
 
 // Launch shape of the sweep kernel for rows of ly nodes: columns per thread,

@@ -170,6 +170,15 @@ class Lattice:
         FP32/FP16, FP64 always 1).  Identical results."""
         check(_lib.load().lbm_set_sweep_tile(self._h, int(threads), int(cols_per_thread)))
 
+    def set_row_layout(self, interleave: bool = True) -> None:
+        """Long two-step runs (64 steps or more) sweep the state in a row order
+        that stores the rows the blocks touch at once next to each other,
+        moving it there and back around the run (default on; used for FP32
+        rows of more than 1024 columns and FP64 column tiles with 16 or more
+        rows per block, where it measured faster).  Identical results; the
+        state is in the plain order between runs."""
+        check(_lib.load().lbm_set_row_layout(self._h, 1 if interleave else 0))
+
     # -- state ------------------------------------------------------------------
     def _shape_check(self, f=None, rho=None, u=None):
         if f is not None:

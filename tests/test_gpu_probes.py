@@ -31,8 +31,9 @@ def test_copy_probe_copies():
     assert L.lbm_copy_probe(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), 20, ctypes.c_void_p(s)) != 0
 
 
-@pytest.mark.parametrize("lx,pitch,rows", [(37, 128, 8), (64, 2048, 32), (9, 2304, 4), (5, 64, 32)])
-def test_row_sweep_probe_writes_every_row_once(lx, pitch, rows):
+@pytest.mark.parametrize("interleaved", [0, 1])
+@pytest.mark.parametrize("lx,pitch,rows", [(37, 128, 8), (64, 2048, 32), (9, 2304, 4), (5, 64, 32), (70, 2048, 34)])
+def test_row_sweep_probe_writes_every_row_once(lx, pitch, rows, interleaved):
     import torch
 
     lib, L = _lib()
@@ -40,8 +41,8 @@ def test_row_sweep_probe_writes_every_row_once(lx, pitch, rows):
     src = torch.from_numpy(rng.standard_normal(9 * lx * pitch).astype(np.float32)).cuda()
     dst = torch.full_like(src, float("nan"))
     s = torch.cuda.current_stream().cuda_stream
-    lib.check(L.lbm_row_sweep_probe(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()), lx, pitch, rows,
-                                    ctypes.c_void_p(s)))
+    lib.check(L.lbm_row_sweep_probe_ordered(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()), lx,
+                                            pitch, rows, interleaved, ctypes.c_void_p(s)))
     torch.cuda.synchronize()
     x = src.view(-1, 2)
     want = torch.stack([torch.addcmul(torch.full_like(x[:, 0], 0.5), x[:, 0], torch.full_like(x[:, 0], 1.0001)),
