@@ -1049,6 +1049,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     }
     // tables: sweep jobs per bucket and pair parity with their block tables,
     // then the one-step jobs' records for the two steps of a pair
+    std::vector<std::pair<int, uint32_t>> ilv;  // (job, rows) in the interleaved row order
     std::vector<SweepGroupTables> gt(buckets.size());
     for (size_t bi = 0; bi < buckets.size(); bi++) {
         Bucket &bk = buckets[bi];
@@ -1081,8 +1082,26 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             const uint32_t ncl = c->bc.view.chunk ? c->bc.view.nclasses : 0;
             bk.lds = std::max(bk.lds, sweep_lds_bytes(c->storage, c->g.ly, ncl, rows));
             if (bk.k == 3) bk.lds_k = std::max(bk.lds_k, sweepk_lds_bytes(c->storage, c->g.ly, ncl, rows, 3));
+            // FP32 rows of two columns per thread in a long run take the
+            // range-interleaved row order, as in lbm_run: moved there now (on
+            // the batch stream, ahead of every launch of the run) and back
+            // after the pairs
+            RowLayout il;
+            if (bk.k == 2 && bk.v == 2 && c->storage == LBM_STORE_F32 && c->cfg.row_interleave &&
+                nsteps / 2 >= kInterleaveMinPairs && rows >= (uint32_t)kInterleaveMinRows && rows < c->g.lx &&
+                c->g.lx < kRowLayoutMaxLx) {
+                il = row_layout(c->g.lx, rows);
+                LBM_HIP_CHECK(launch_row_layout(c->storage, c->g, c->st, c->cur, rows, true, b->stream),
+                              "lbm_batch_run: row order kernel");
+                c->cur ^= 1;
+                ilv.push_back({bk.members[m], rows});
+            }
             for (int pp = 0; pp < 2; pp++) {
                 SweepJob d;
+                d.il_rows = il.rows;
+                d.il_ranges = il.ranges;
+                d.il_whole = il.whole;
+                d.il_magic = il.magic;
                 const int src = c->cur ^ pp;
                 d.src = c->st.plane[src];
                 d.dst = c->st.plane[src ^ 1];
@@ -1379,6 +1398,13 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     }
     for (int j : pjobs) b->ctxs[j]->cur ^= (int)(pairs & 1);
     for (int j = 0; j < n; j++) b->ctxs[j]->steps += 2 * pairs;
+    // back to the plain row order (after the join: on the batch stream)
+    for (const auto &iv : ilv) {
+        lbm_ctx *c = b->ctxs[iv.first];
+        LBM_HIP_CHECK(launch_row_layout(c->storage, c->g, c->st, c->cur, iv.second, false, b->stream),
+                      "lbm_batch_run: row order kernel");
+        c->cur ^= 1;
+    }
     if (nsteps & 1) {
         // the odd step, job by job
         for (int j = 0; j < n; j++) {

@@ -1297,6 +1297,14 @@ __global__ __launch_bounds__(kSweepThreads, sizeof(S) < 8 && V == 1 && !kBig ? 8
     p.scratch = static_cast<S *>(j.scratch);
     p.ntiles = 1;
     p.tw = p.lw = j.g.ly;
+    // only two columns per thread take the interleaved row order (the
+    // one-column instances keep the plain one at no register cost)
+    if constexpr (V == 2) {
+        p.il.rows = j.il_rows;
+        p.il.ranges = j.il_ranges;
+        p.il.whole = j.il_whole;
+        p.il.magic = j.il_magic;
+    }
     sweep_block<S, V, true, false, kBig>(p, (blk - j.block0) * j.rows, 0, SweepLanes{threadIdx.x, blockDim.x, smem, 0});
 }
 

@@ -112,7 +112,9 @@ struct SweepJob {
     int64_t probe_offset = 0;  // step-1 ring slot = batch step counter + probe_offset
     uint32_t rows = 0;         // output rows per block
     uint32_t block0 = 0;       // first block of the job in its bucket's grid
-    uint32_t pad = 0;
+    // row order of the state buffers (lbm_rowlayout.h; il_rows = 0: plain)
+    uint32_t il_rows = 0, il_ranges = 0, il_whole = 0;
+    uint64_t il_magic = 0;
 };
 
 // Packed sweep launch: each block of kPackedWaves waves sweeps several row

@@ -94,3 +94,34 @@ def test_row_layout_argument_checked():
     with pytest.raises(LbmError, match="interleave"):
         check(load().lbm_set_row_layout(lat._h, 2))
     lat.close()
+
+
+def test_batch_groups_of_wide_rows_in_interleaved_order_equal_single_runs():
+    """A batch whose FP32 lattices of more than 1024 columns (two columns per
+    thread) sweep long runs in the interleaved row order, beside a narrow
+    lattice in the plain order: every lattice equals its own plain-order run
+    of single steps, over runs of 64 steps and more (pending start, odd
+    steps) and a short one."""
+    from lattice_boltzmann_parallel_solver_amd import Batch
+
+    shapes = [(300, 1100, "channel"), (257, 2048, "couette"), (100, 1500, "channel"), (90, 300, "channel")]
+    chunks = (1, 64, 3, 71)
+    lats, refs = [], []
+    for k, (lx, ly, kind) in enumerate(shapes):
+        a, _ = _lattice(lx, ly, "f32", kind, True, 0, seed=11 + k, probe=(lx // 3, ly // 2), steps=sum(chunks))
+        r, _ = _lattice(lx, ly, "f32", kind, False, 0, seed=11 + k, probe=(lx // 3, ly // 2), steps=sum(chunks))
+        r.set_temporal(1, 0)
+        lats.append(a)
+        refs.append(r)
+    b = Batch(lats, [1.3] * len(lats), temporal=2, rows_per_block=16)
+    for chunk in chunks:
+        b.run(chunk)
+        for r in refs:
+            r.run(1.3, chunk)
+    b.close()
+    for a, r, (lx, ly, _) in zip(lats, refs, shapes):
+        for x, y in zip(a.state(), r.state()):
+            assert np.array_equal(x, y), (lx, ly)
+        assert np.array_equal(a.probe(), r.probe()), (lx, ly)
+        a.close()
+        r.close()
