@@ -483,21 +483,25 @@ int lbm_run(lbm_ctx *c, double omega, int64_t nsteps) {
                 ilv = true;
             }
         }
+        hipError_t e = hipSuccess;
         for (; left >= 2; left -= 2) {
             // materialises f_{steps} and f_{steps+1}
             const int64_t slot = probing ? c->steps - c->probe_base - 1 : -1;
-            LBM_HIP_CHECK(launch_sweep2(c->storage, c->g, c->st, c->cur, c->bc.view, omega, c->probe, slot,
-                                        probing ? slot + 1 : -1, cfg, c->stream),
-                          "two-step sweep kernel");
+            e = launch_sweep2(c->storage, c->g, c->st, c->cur, c->bc.view, omega, c->probe, slot,
+                              probing ? slot + 1 : -1, cfg, c->stream);
+            if (e != hipSuccess) break;
             c->cur ^= 1;
             c->steps += 2;
         }
+        // back to the plain order, also after a failed launch (the state then
+        // holds the steps done so far)
         if (ilv) {
-            LBM_HIP_CHECK(launch_row_layout(c->storage, c->g, c->st, c->cur, (uint32_t)cfg.sweep_rows, false,
-                                            c->stream),
-                          "row order kernel");
-            c->cur ^= 1;
+            const hipError_t eb = launch_row_layout(c->storage, c->g, c->st, c->cur, (uint32_t)cfg.sweep_rows, false,
+                                                    c->stream);
+            if (eb == hipSuccess) c->cur ^= 1;
+            LBM_HIP_CHECK(eb, "row order kernel");
         }
+        LBM_HIP_CHECK(e, "two-step sweep kernel");
     }
     for (; left > 0; left--) {
         // this launch materialises f_{steps} in registers: record it if probed
