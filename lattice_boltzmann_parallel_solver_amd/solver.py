@@ -156,11 +156,13 @@ class Lattice:
         check(_lib.load().lbm_set_launch(self._h, int(cells_per_thread), int(bool(xcd_remap)), int(bool(nt_stores))))
 
     def set_temporal(self, steps_per_launch: int = 2, rows_per_block: int = 0) -> None:
-        """Advance `steps_per_launch` (1 or 2) steps per kernel launch; two steps
-        keep the intermediate state on chip (identical results, half the
-        population traffic).  Rows wider than one kernel block run as column
-        tiles.  Raises LbmError where the two-step kernel does not apply
-        (pressure fix, fewer than three rows, more than 128 boundary classes)."""
+        """Advance `steps_per_launch` (1 to 4) steps per kernel launch; two or
+        more steps keep the intermediate states on chip (identical results,
+        1/K of the population traffic per step).  Rows wider than one kernel
+        block run as column tiles with two steps; three and four need FP32 or
+        FP16 rows of at most 1024 columns.  Raises LbmError where the kernel
+        does not apply (pressure fix, fewer than three rows, more than 128
+        boundary classes, LDS)."""
         check(_lib.load().lbm_set_temporal(self._h, int(steps_per_launch), int(rows_per_block)))
 
     def set_sweep_tile(self, threads: int = 0, cols_per_thread: int = 0) -> None:
