@@ -186,7 +186,8 @@ int lbm_set_sweep_tile(lbm_ctx *ctx, int threads, int cols_per_thread);
  * other (so the rows that all blocks read and write at once are neighbours in
  * memory), sweeps there, and copies it back before returning -- for FP32 rows
  * of two columns per thread (ny > 1024) and FP64 column tiles (ny > 1024)
- * with 16 or more rows per block, where it measured faster; 0: the plain row
+ * with 16 or more rows per block and at most two rounds of the row ranges
+ * the GPU holds at once, where it measured faster; 0: the plain row
  * order throughout.  Results are identical; the state is in the plain order
  * whenever lbm_run has returned.  LBM_ERR_ARG for other values.  No
  * counterpart in the reference (a layout setting of this library). */

@@ -463,18 +463,23 @@ int lbm_run(lbm_ctx *c, double omega, int64_t nsteps) {
         // the sweep's row ranges (the rows all blocks touch at once become
         // neighbours in memory) and back at its end: two copies of the state.
         // Measured where it pays: FP32 rows of two columns per thread and FP64
-        // column tiles, with 16 or more rows per block (8192 x 2048 FP32
-        // 137.3 -> 147.4 GLUPS, FP64 67.7 -> 69.4, 4200 x 1800 FP32 125.5 ->
-        // 128.3).  It lost on one column per thread (8192 x 1024 FP32 130.1 ->
-        // 121.5), at 8 rows per block (2048 x 2048 FP32 125.7 -> 122.9) and in
-        // FP16 (220.5 -> 214.4), which keep the plain order.
+        // column tiles, with 16 or more rows per block and at most two rounds
+        // of resident row ranges (8192 x 2048 FP32 137.3 -> 147.4 GLUPS, FP64
+        // 67.7 -> 69.4, 4200 x 1800 FP32 125.5 -> 128.3, 65536 x 2048 FP32 in
+        // two rounds 141.8 -> 144.1).  It lost on one column per thread (8192
+        // x 1024 FP32 130.1 -> 121.5), at 8 rows per block (2048 x 2048 FP32
+        // 125.7 -> 122.9), in FP16 (220.5 -> 214.4) and over seven rounds
+        // (65536 x 2048 FP64 69.1 -> 65.1), which keep the plain order.
         LaunchCfg cfg = c->cfg;
         bool ilv = false;
         const SweepShape sh = sweep_shape(c->storage, c->g.ly, cfg.sweep_tile, cfg.sweep_v);
         if (cfg.row_interleave && left / 2 >= kInterleaveMinPairs && c->g.lx < kRowLayoutMaxLx &&
             c->storage != LBM_STORE_F16 && (sh.v == 2 || sh.ntiles > 1)) {
             const int rows = sweep2_rows(c->storage, c->g, c->bc.view, cfg);
-            if (rows >= kInterleaveMinRows && rows <= (int)kRowLayoutMaxRows && rows < (int)c->g.lx) {
+            const int64_t ranges = rows > 0 ? ((int64_t)c->g.lx + rows - 1) / rows : 0;
+            const int resident = sweep2_resident_ranges(c->storage, c->g, c->bc.view, cfg);
+            if (rows >= kInterleaveMinRows && rows <= (int)kRowLayoutMaxRows && rows < (int)c->g.lx &&
+                resident > 0 && ranges <= 2 * (int64_t)resident) {
                 LBM_HIP_CHECK(launch_row_layout(c->storage, c->g, c->st, c->cur, (uint32_t)rows, true, c->stream),
                               "row order kernel");
                 c->cur ^= 1;

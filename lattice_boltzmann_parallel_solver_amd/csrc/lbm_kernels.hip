@@ -2167,26 +2167,30 @@ const void *sweep_fn(bool probing, bool tiled, bool big) {
     return probing ? sweep_fn_p<S, VV, true>(tiled, big) : sweep_fn_p<S, VV, false>(tiled, big);
 }
 
-// Output rows per block of the two-step launch for this lattice and shape.
+// Row ranges the GPU holds at once for the two-step launch of this lattice
+// and shape (resident blocks per CU, from LDS and registers, times the CU
+// count, divided by the column tiles of a range); 0 if the runtime cannot say.
 template <class S>
-uint32_t sweep_rows_typed(const Geom &g, const BcView &bc, const LaunchCfg &cfg) {
+int sweep_resident_ranges(const Geom &g, const BcView &bc, const LaunchCfg &cfg) {
     const SweepShape sh = sweep_shape(sizeof(S) == 8 ? LBM_STORE_F64 : (sizeof(S) == 4 ? LBM_STORE_F32 : LBM_STORE_F16),
                                       g.ly, cfg.sweep_tile, cfg.sweep_v);
     const bool big = sweep_big(sizeof(S) == 8 ? LBM_STORE_F64 : (sizeof(S) == 4 ? LBM_STORE_F32 : LBM_STORE_F16), g);
     constexpr int V2 = sizeof(S) == 8 ? 1 : 2;
-    int slots = 0;
-    if (cfg.sweep_rows <= 0) {
-        // resident blocks per CU (LDS and registers) times the CU count, in
-        // row ranges (the tiles of a range are separate blocks)
-        int dev = 0, cus = 0, per_cu = 0;
-        const size_t lds_est = sweep_lds_layout<S>(sh.lw, bc.chunk ? bc.nclasses : 0, 64).total;
-        const void *fn = sh.v == 1 ? sweep_fn<S, 1>(false, sh.ntiles > 1, big) : sweep_fn<S, V2>(false, sh.ntiles > 1, big);
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_est) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)sh.threads, lds_est) == hipSuccess)
-            slots = std::max(1, cus * (per_cu > 0 ? per_cu : 1) / (int)sh.ntiles);
-    }
+    int dev = 0, cus = 0, per_cu = 0;
+    const size_t lds_est = sweep_lds_layout<S>(sh.lw, bc.chunk ? bc.nclasses : 0, 64).total;
+    const void *fn = sh.v == 1 ? sweep_fn<S, 1>(false, sh.ntiles > 1, big) : sweep_fn<S, V2>(false, sh.ntiles > 1, big);
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_est) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)sh.threads, lds_est) == hipSuccess)
+        return std::max(1, cus * (per_cu > 0 ? per_cu : 1) / (int)sh.ntiles);
+    return 0;
+}
+
+// Output rows per block of the two-step launch for this lattice and shape.
+template <class S>
+uint32_t sweep_rows_typed(const Geom &g, const BcView &bc, const LaunchCfg &cfg) {
+    const int slots = cfg.sweep_rows <= 0 ? sweep_resident_ranges<S>(g, bc, cfg) : 0;
     return (uint32_t)sweep_rows_for(g, cfg, slots);
 }
 
@@ -2320,6 +2324,15 @@ int sweep2_rows(int storage, const Geom &g, const BcView &bc, const LaunchCfg &c
         case LBM_STORE_F64: return (int)sweep_rows_typed<double>(g, bc, cfg);
         case LBM_STORE_F32: return (int)sweep_rows_typed<float>(g, bc, cfg);
         case LBM_STORE_F16: return (int)sweep_rows_typed<__half>(g, bc, cfg);
+    }
+    return 0;
+}
+
+int sweep2_resident_ranges(int storage, const Geom &g, const BcView &bc, const LaunchCfg &cfg) {
+    switch (storage) {
+        case LBM_STORE_F64: return sweep_resident_ranges<double>(g, bc, cfg);
+        case LBM_STORE_F32: return sweep_resident_ranges<float>(g, bc, cfg);
+        case LBM_STORE_F16: return sweep_resident_ranges<__half>(g, bc, cfg);
     }
     return 0;
 }

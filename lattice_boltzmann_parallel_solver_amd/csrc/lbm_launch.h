@@ -30,6 +30,8 @@ hipError_t launch_sweep2(int storage, const Geom &g, const StateView &st, int sr
                          const LaunchCfg &cfg, hipStream_t s);
 // Output rows per block that launch_sweep2 uses with this configuration.
 int sweep2_rows(int storage, const Geom &g, const BcView &bc, const LaunchCfg &cfg);
+// Row ranges of the two-step launch the GPU holds at once (0: unknown).
+int sweep2_resident_ranges(int storage, const Geom &g, const BcView &bc, const LaunchCfg &cfg);
 // planes[src] -> planes[src^1] and side[src] -> side[src^1], rows moved to
 // (to_ilv) or from their slots in the range-interleaved order of `rows`-row
 // ranges (lbm_rowlayout.h); launch_sweep2 with cfg.sweep_il and cfg.sweep_rows
