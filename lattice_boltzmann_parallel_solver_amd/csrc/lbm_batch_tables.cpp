@@ -112,4 +112,27 @@ bool plan_packed_blocks(const std::vector<PackItem> &items, uint32_t block_waves
     return true;
 }
 
+int pick_wide_rows(const std::vector<WideLattice> &lattices, int cus, int max_rows, int halo) {
+    if (lattices.empty() || cus < 1 || max_rows < 1) return 0;
+    // exact arithmetic: CU shares in units of 1 / 840 (every per_cu of 1..8 divides 840)
+    constexpr uint64_t kUnit = 840;
+    int best_r = 0;
+    uint64_t best = 0;
+    for (int r = 1; r <= max_rows; r++) {
+        uint64_t share = 0;
+        for (const WideLattice &w : lattices) {
+            const uint64_t per_cu = (uint64_t)(w.per_cu >= 1 && w.per_cu <= 8 ? w.per_cu : 1);
+            share += ((uint64_t)w.lx + (uint64_t)r - 1) / (uint64_t)r * (kUnit / per_cu);
+        }
+        const uint64_t per_round = (uint64_t)cus * kUnit;
+        const uint64_t rounds = (share + per_round - 1) / per_round;
+        const uint64_t cost = rounds * (uint64_t)(r + halo);
+        if (best_r == 0 || cost < best) {
+            best = cost;
+            best_r = r;
+        }
+    }
+    return best_r;
+}
+
 }  // namespace lbm

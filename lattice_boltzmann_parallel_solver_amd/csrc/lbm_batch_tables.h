@@ -67,4 +67,18 @@ struct PackItem {
 bool plan_packed_blocks(const std::vector<PackItem> &items, uint32_t block_waves, uint32_t lds_budget,
                         PackedTables &tables);
 
+// One lattice of the wide two-step groups of a batch: its rows and how many
+// blocks of its group one CU holds at once (1 to 8).
+struct WideLattice {
+    uint32_t lx = 0;
+    int per_cu = 1;
+};
+
+// Rows per block shared by the wide groups: the r in 1..max_rows that
+// minimises rounds(r) x (r + halo), where rounds(r) = ceil(sum of
+// ceil(lx / r) / per_cu over the lattices, over cus) is the number of rounds
+// of resident blocks the launches take; ties go to the smaller r.  0 when
+// there is no lattice or cus < 1.
+int pick_wide_rows(const std::vector<WideLattice> &lattices, int cus, int max_rows, int halo);
+
 }  // namespace lbm

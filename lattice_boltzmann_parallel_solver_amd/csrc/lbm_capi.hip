@@ -1021,8 +1021,8 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
     // that minimises rounds x (rows + 3) over all their blocks together (a
     // block also reads and computes about three halo rows).  Measured, FP32
     // GLUPS at 32 rows / at the chosen rows: four 4200 x 1800 channels 108.3 /
-    // 131.8 (33), one 2048 x 2048 lattice 50.7 / 115.2 (8), two 8192 x 2048
-    // channels with a 2048 and a 1500 square 124.0 / 133.1 (40), one 8192 x
+    // 131.8 (66), one 2048 x 2048 lattice 50.7 / 115.2 (8), two 8192 x 2048
+    // channels with a 2048 and a 1500 square 124.0 / 133.1 (79), one 8192 x
     // 2048 channel 133.0 (32 chosen; 34 gives 136.3).  Where the wide groups
     // hold less than half of the swept nodes their blocks share the GPU with
     // the narrow groups' and the rule for those applies to them too.
@@ -1034,7 +1034,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             if (bk.k == 2 && bk.threads >= 512) wide_nodes += b->ctxs[j]->g.N;
         }
     if (b->rows_auto && 2 * wide_nodes >= swept_nodes) {
-        std::vector<std::pair<uint32_t, int>> wide;  // (lx, blocks per CU)
+        std::vector<WideLattice> wide;
         for (const Bucket &bk : buckets) {
             if (bk.k != 2 || bk.threads < 512) continue;
             size_t lds64 = 0;
@@ -1043,18 +1043,9 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
                 lds64 = std::max(lds64, sweep_lds_bytes(c->storage, c->g.ly, c->bc.view.chunk ? c->bc.view.nclasses : 0, 64));
             }
             const int per_cu = sweep2_batch_per_cu(b->storage, bk.v, bk.big, bk.threads, lds64);
-            for (int j : bk.members) wide.emplace_back(b->ctxs[j]->g.lx, per_cu);
+            for (int j : bk.members) wide.push_back({b->ctxs[j]->g.lx, per_cu});
         }
-        double best = 0;
-        for (uint32_t r = 1; r <= 128 && !wide.empty(); r++) {
-            double slots = 0;  // CU share of the wide blocks at r rows per block
-            for (const auto &w : wide) slots += (double)((w.first + r - 1) / r) / w.second;
-            const double t = std::ceil(slots / cus - 1e-9) * (r + 3);
-            if (wide_rows == 0 || t < best) {
-                best = t;
-                wide_rows = (int)r;
-            }
-        }
+        wide_rows = pick_wide_rows(wide, cus, 128, 3);
     }
     // tables: sweep jobs per bucket and pair parity with their block tables,
     // then the one-step jobs' records for the two steps of a pair
