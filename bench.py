@@ -21,11 +21,12 @@ roofline   : HBM bound; achieved = algorithmic bytes per launch (every populatio
              1 GiB streaming copy (16 bytes per lane) timed the same way, the
              practical ceiling, and frac_of_copy = achieved / copy_gbs.
              row_sweep_pattern_gbs: the two-step kernel's access pattern with no
-             work (lbm_row_sweep_probe_ordered) at the headline's rows per
-             block, in the range-interleaved row order its long runs use: the
-             ceiling of a row sweep on this layout, and
-             frac_of_row_sweep_pattern = achieved / that;
-             row_sweep_pattern_plain_order_gbs: the same in the plain order.
+             work (lbm_row_sweep_probe_ordered: the pull's rows, one row
+             prefetched, a barrier per row) at the headline's rows per block,
+             in the range-interleaved row order its long runs use: the ceiling
+             of a row sweep on this layout, and frac_of_row_sweep_pattern =
+             achieved / that; row_sweep_pattern_plain_order_gbs: the same in
+             the plain order.
 single_step: the same workload with one step per launch (k_fused); its
              periodic_lattice entry times the same kernel on a fully periodic
              8192x2048 lattice (no boundary nodes).
@@ -150,10 +151,10 @@ def copy_gbs(torch, nbytes=1 << 30, reps=10):
 def row_sweep_gbs(torch, rows=32, reps=10, interleaved=True):
     """HBM bandwidth of the two-step kernel's access pattern with no work
     (the bench lattice's shape: nine planes of 2048 FP32 values per row,
-    `rows` rows per block, one row prefetched, a barrier per row), read +
-    write counted, with the rows in the range-interleaved order of long
-    two-step runs or in the plain order: the ceiling a row sweep reaches on
-    this layout."""
+    `rows` rows per block, each plane read from the row the pull reads it
+    from, one row prefetched, a barrier per row), read + write counted, with
+    the rows in the range-interleaved order of long two-step runs or in the
+    plain order: the ceiling a row sweep reaches on this layout."""
     import ctypes
 
     from lattice_boltzmann_parallel_solver_amd import _lib
@@ -165,8 +166,9 @@ def row_sweep_gbs(torch, rows=32, reps=10, interleaved=True):
     L = _lib.load()
 
     def once():
+        mode = 2 | (1 if interleaved else 0)  # the pull's rows, in the chosen row order
         _lib.check(L.lbm_row_sweep_probe_ordered(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), LX,
-                                                 pitch, rows, 1 if interleaved else 0, ctypes.c_void_p(stream)))
+                                                 pitch, rows, mode, ctypes.c_void_p(stream)))
 
     once()
     torch.cuda.synchronize()

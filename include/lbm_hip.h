@@ -252,13 +252,18 @@ int lbm_copy_probe(const void *src, void *dst, int64_t bytes, void *stream);
  * on `stream`.  The bandwidth ceiling of a row sweep, beside the copy's.
  * pitch even, at most 4096.  No counterpart in the reference. */
 int lbm_row_sweep_probe(const void *src, void *dst, int nx, int pitch, int rows, void *stream);
-/* lbm_row_sweep_probe with the rows in the plain order (interleaved = 0) or
- * in the range-interleaved order of long two-step runs (1: offset t of every
- * `rows`-row range stored together, lbm_set_row_layout; rows <= 128, nx <
- * 2^24).  Every value of src is transformed into the same place of dst in
- * either order.  No counterpart in the reference. */
-int lbm_row_sweep_probe_ordered(const void *src, void *dst, int nx, int pitch, int rows, int interleaved,
-                                void *stream);
+/* lbm_row_sweep_probe with options, `mode` a combination of:
+ * LBM_PROBE_INTERLEAVED: rows in the range-interleaved order of long
+ *   two-step runs (offset t of every `rows`-row range stored together,
+ *   lbm_set_row_layout; rows <= 128, nx < 2^24), else the plain order;
+ * LBM_PROBE_PULL: plane i of output row x read from row x - cx_i (periodic
+ *   in x, the D2Q9 velocities), the rows the pull reads, else from row x.
+ * dst row x plane i = the transformed src row (x or x - cx_i) plane i, each
+ * at its row's place in the chosen order.  LBM_ERR_ARG for other modes.  No
+ * counterpart in the reference. */
+#define LBM_PROBE_INTERLEAVED 1
+#define LBM_PROBE_PULL 2
+int lbm_row_sweep_probe_ordered(const void *src, void *dst, int nx, int pitch, int rows, int mode, void *stream);
 
 /* ---- reductions that allocate nothing ----------------------------------- */
 /* Largest ncells lbm_moments_sum_noalloc accepts (174762 blocks of 256). */

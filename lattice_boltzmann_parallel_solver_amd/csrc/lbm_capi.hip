@@ -1686,20 +1686,20 @@ int lbm_copy_probe(const void *src, void *dst, int64_t bytes, void *stream) {
     return LBM_OK;
 }
 
-int lbm_row_sweep_probe_ordered(const void *src, void *dst, int nx, int pitch, int rows, int interleaved,
-                                void *stream) {
-    if (nx <= 0 || rows <= 0 || pitch <= 0 || pitch % 2 != 0 || pitch > 4096 || !src || !dst ||
-        (interleaved != 0 && interleaved != 1)) {
+int lbm_row_sweep_probe_ordered(const void *src, void *dst, int nx, int pitch, int rows, int mode, void *stream) {
+    if (nx <= 0 || rows <= 0 || pitch <= 0 || pitch % 2 != 0 || pitch > 4096 || !src || !dst || mode < 0 ||
+        mode > 3) {
         set_error("lbm_row_sweep_probe: need device buffers, nx > 0, rows > 0, an even pitch up to 4096 and "
-                  "interleaved 0 or 1");
+                  "mode 0 to 3");
         return LBM_ERR_ARG;
     }
+    const bool interleaved = (mode & LBM_PROBE_INTERLEAVED) != 0;
     if (interleaved && (rows > (int)kRowLayoutMaxRows || (uint32_t)nx >= kRowLayoutMaxLx)) {
         set_error("lbm_row_sweep_probe: the interleaved order needs rows <= 128 and nx < 2^24");
         return LBM_ERR_ARG;
     }
-    LBM_HIP_CHECK(launch_row_sweep_probe(src, dst, (uint32_t)nx, (uint32_t)pitch, (uint32_t)rows, interleaved != 0,
-                                         static_cast<hipStream_t>(stream)),
+    LBM_HIP_CHECK(launch_row_sweep_probe(src, dst, (uint32_t)nx, (uint32_t)pitch, (uint32_t)rows, interleaved,
+                                         (mode & LBM_PROBE_PULL) != 0, static_cast<hipStream_t>(stream)),
                   "lbm_row_sweep_probe");
     return LBM_OK;
 }

@@ -2626,8 +2626,10 @@ __global__ __launch_bounds__(kBlock) void k_copy16(const uint4 *__restrict__ src
 // R) of the row-interleaved layout (nine planes of pitch FP32 values per row),
 // loading the next row before it writes the current one, a barrier per row,
 // one multiply-add per value; blocks in XCD order.  NS float2 column slots per
-// lane (pitch / 2 <= NS blockDim.x).
-template <int NS>
+// lane (pitch / 2 <= NS blockDim.x).  kPull: plane i of output row x is read
+// from row x - cx_i (periodic), as the pull reads it; else from row x.  Rows
+// live at their slots of layout L.
+template <int NS, bool kPull>
 __global__ __launch_bounds__(1024) void k_row_sweep_probe(const float2 *__restrict__ src, float2 *__restrict__ dst,
                                                           uint32_t lx, uint32_t pitch2, uint32_t rows, RowLayout L) {
     const uint32_t xa = xcd_block(blockIdx.x, gridDim.x) * rows;
@@ -2636,20 +2638,22 @@ __global__ __launch_bounds__(1024) void k_row_sweep_probe(const float2 *__restri
     uint32_t col[NS];
 #pragma unroll
     for (int k = 0; k < NS; k++) col[k] = min(threadIdx.x + k * T, pitch2 - 1);
+    auto load = [&](uint32_t x, float2 out[NS][9]) {
+        const size_t s0 = row_slot(L, x);
+        const size_t sm = kPull ? row_slot(L, x == 0 ? lx - 1 : x - 1) : s0;
+        const size_t sp = kPull ? row_slot(L, x + 1 == lx ? 0 : x + 1) : s0;
+#pragma unroll
+        for (int k = 0; k < NS; k++)
+#pragma unroll
+            for (int i = 0; i < 9; i++) {
+                const size_t sr = kCX[i] == 1 ? sm : (kCX[i] == -1 ? sp : s0);
+                out[k][i] = src[(sr * 9 + i) * pitch2 + col[k]];
+            }
+    };
     float2 cur[NS][9], nxt[NS][9];
-    const size_t s0 = row_slot(L, xa);
-#pragma unroll
-    for (int k = 0; k < NS; k++)
-#pragma unroll
-        for (int i = 0; i < 9; i++) cur[k][i] = src[(s0 * 9 + i) * pitch2 + col[k]];
+    load(xa, cur);
     for (uint32_t x = xa; x < xe; x++) {
-        if (x + 1 < xe) {
-            const size_t s1 = row_slot(L, x + 1);
-#pragma unroll
-            for (int k = 0; k < NS; k++)
-#pragma unroll
-                for (int i = 0; i < 9; i++) nxt[k][i] = src[(s1 * 9 + i) * pitch2 + col[k]];
-        }
+        if (x + 1 < xe) load(x + 1, nxt);
         // lanes past the last slot write the last one again (same values)
         const size_t sx = row_slot(L, x);
 #pragma unroll
@@ -2667,7 +2671,7 @@ __global__ __launch_bounds__(1024) void k_row_sweep_probe(const float2 *__restri
 }
 
 hipError_t launch_row_sweep_probe(const void *src, void *dst, uint32_t lx, uint32_t pitch, uint32_t rows,
-                                  bool interleaved, hipStream_t s) {
+                                  bool interleaved, bool pull, hipStream_t s) {
     if (lx == 0 || rows == 0 || pitch == 0 || pitch % 2 != 0 || pitch > 4096) return hipErrorInvalidValue;
     RowLayout L;
     if (interleaved) {
@@ -2676,13 +2680,19 @@ hipError_t launch_row_sweep_probe(const void *src, void *dst, uint32_t lx, uint3
     }
     const uint32_t pitch2 = pitch / 2;
     const unsigned grid = (lx + rows - 1) / rows;
+    const auto *in = static_cast<const float2 *>(src);
+    auto *out = static_cast<float2 *>(dst);
     if (pitch2 <= 1024) {
         const uint32_t threads = (pitch2 + 63) / 64 * 64;
-        hipLaunchKernelGGL(k_row_sweep_probe<1>, dim3(grid), dim3(threads), 0, s, static_cast<const float2 *>(src),
-                           static_cast<float2 *>(dst), lx, pitch2, rows, L);
+        if (pull)
+            hipLaunchKernelGGL((k_row_sweep_probe<1, true>), dim3(grid), dim3(threads), 0, s, in, out, lx, pitch2, rows, L);
+        else
+            hipLaunchKernelGGL((k_row_sweep_probe<1, false>), dim3(grid), dim3(threads), 0, s, in, out, lx, pitch2, rows, L);
     } else {
-        hipLaunchKernelGGL(k_row_sweep_probe<2>, dim3(grid), dim3(1024), 0, s, static_cast<const float2 *>(src),
-                           static_cast<float2 *>(dst), lx, pitch2, rows, L);
+        if (pull)
+            hipLaunchKernelGGL((k_row_sweep_probe<2, true>), dim3(grid), dim3(1024), 0, s, in, out, lx, pitch2, rows, L);
+        else
+            hipLaunchKernelGGL((k_row_sweep_probe<2, false>), dim3(grid), dim3(1024), 0, s, in, out, lx, pitch2, rows, L);
     }
     return hipGetLastError();
 }

@@ -119,9 +119,10 @@ bool sweep_big(int storage, const Geom &g);
 size_t sweep_lds_bytes(int storage, uint32_t ly, uint32_t nclasses, uint32_t rows);
 // The two-step kernel's row-sweep access pattern with no work (measurement
 // only): FP32 buffers of 9 lx pitch values, `rows` rows per block, rows in
-// the plain or the range-interleaved order of those ranges.
+// the plain or the range-interleaved order of those ranges; with `pull` each
+// plane is read from the row the pull reads it from (x - cx_i).
 hipError_t launch_row_sweep_probe(const void *src, void *dst, uint32_t lx, uint32_t pitch, uint32_t rows,
-                                  bool interleaved, hipStream_t s);
+                                  bool interleaved, bool pull, hipStream_t s);
 // dst[i] = src[i] for n16 16-byte chunks (streaming copy, measurement only).
 hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t s);
 // *step += k (one thread).

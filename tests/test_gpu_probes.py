@@ -31,22 +31,52 @@ def test_copy_probe_copies():
     assert L.lbm_copy_probe(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), 20, ctypes.c_void_p(s)) != 0
 
 
-@pytest.mark.parametrize("interleaved", [0, 1])
+CX = [0, 1, 0, -1, 0, 1, -1, -1, 1]
+
+
+def _slot(lx, rows, x, interleaved):
+    """Slot of row x in the range-interleaved order (offset t of every
+    rows-row range together; the offsets the last range lacks packed over the
+    full ranges), or x in the plain order."""
+    if not interleaved:
+        return x
+    ranges = -(-lx // rows)
+    whole = rows if lx % rows == 0 else lx % rows
+    b, t = divmod(x, rows)
+    return t * ranges + b if t < whole else whole * ranges + (t - whole) * (ranges - 1) + b
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("lx,pitch,rows", [(37, 128, 8), (64, 2048, 32), (9, 2304, 4), (5, 64, 32), (70, 2048, 34)])
-def test_row_sweep_probe_writes_every_row_once(lx, pitch, rows, interleaved):
+def test_row_sweep_probe_writes_every_row_once(lx, pitch, rows, mode):
+    """Every output row x, plane i, holds the transformed input of row x (or,
+    with the pull pattern, of row x - cx_i), each at its row's place in the
+    plain or the interleaved order."""
     import torch
 
     lib, L = _lib()
     rng = np.random.default_rng(lx + pitch)
-    src = torch.from_numpy(rng.standard_normal(9 * lx * pitch).astype(np.float32)).cuda()
+    host = rng.standard_normal(9 * lx * pitch).astype(np.float32)
+    src = torch.from_numpy(host).cuda()
     dst = torch.full_like(src, float("nan"))
     s = torch.cuda.current_stream().cuda_stream
     lib.check(L.lbm_row_sweep_probe_ordered(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()), lx,
-                                            pitch, rows, interleaved, ctypes.c_void_p(s)))
+                                            pitch, rows, mode, ctypes.c_void_p(s)))
     torch.cuda.synchronize()
-    x = src.view(-1, 2)
-    want = torch.stack([torch.addcmul(torch.full_like(x[:, 0], 0.5), x[:, 0], torch.full_like(x[:, 0], 1.0001)),
-                        torch.addcmul(torch.full_like(x[:, 1], 0.25), x[:, 1], torch.full_like(x[:, 1], 0.9999))], 1)
-    assert torch.allclose(dst.view(-1, 2), want, rtol=1e-6, atol=1e-6)
+    il, pull = bool(mode & 1), bool(mode & 2)
+    rows_in = host.reshape(lx, 9, pitch).astype(np.float64)
+    want = np.empty_like(rows_in)
+    for x in range(lx):
+        for i in range(9):
+            xs = (x - CX[i]) % lx if pull else x
+            v = rows_in[_slot(lx, rows, xs, il), i]
+            w = np.empty_like(v)
+            w[0::2] = v[0::2] * np.float32(1.0001) + 0.5
+            w[1::2] = v[1::2] * np.float32(0.9999) + 0.25
+            want[_slot(lx, rows, x, il), i] = w
+    got = dst.cpu().numpy().reshape(lx, 9, pitch)
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-6)
+    assert L.lbm_row_sweep_probe_ordered(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()), lx, pitch,
+                                         rows, 4, ctypes.c_void_p(s)) != 0
     assert L.lbm_row_sweep_probe(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()), lx, 63, rows,
                                  ctypes.c_void_p(s)) != 0
