@@ -135,4 +135,20 @@ int pick_wide_rows(const std::vector<WideLattice> &lattices, int cus, int max_ro
     return best_r;
 }
 
+int batch_group_rows(int threads, int k, int base_rows, bool rows_auto, int64_t rows_total, int cus, int wide_rows) {
+    int rows = threads >= 512 ? std::max(base_rows, 32) : base_rows;
+    if (k == 3) rows = std::max(rows, 4 * base_rows);
+    if (rows_auto) {
+        if (k == 2 && threads >= 512 && wide_rows > 0) {
+            rows = wide_rows;
+        } else {
+            const int64_t slots = (int64_t)(threads >= 512 ? 2 : 4) * std::max(cus, 1);
+            rows = (int)std::min<int64_t>(rows, std::max<int64_t>(1, rows_total / slots));
+        }
+    }
+    return rows;
+}
+
+bool batch_three_steps_fill(int64_t rows_total, int cus) { return rows_total / (4 * (int64_t)std::max(cus, 1)) >= 8; }
+
 }  // namespace lbm

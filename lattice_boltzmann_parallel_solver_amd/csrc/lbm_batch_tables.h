@@ -81,4 +81,16 @@ struct WideLattice {
 // there is no lattice or cus < 1.
 int pick_wide_rows(const std::vector<WideLattice> &lattices, int cus, int max_rows, int halo);
 
+// Rows per block asked of a group's launches (before the kernels' own
+// clamps): base_rows (the batch's setting) for rows of fewer than 512
+// threads, at least 32 for wider ones, at least 4 base_rows for three steps
+// per launch (k = 3).  With automatic rows (rows_auto), wide two-step groups
+// take wide_rows where it is set, and the others no more than leave the
+// batch's rows_total rows about four blocks per CU (two for wide rows).
+int batch_group_rows(int threads, int k, int base_rows, bool rows_auto, int64_t rows_total, int cus, int wide_rows);
+
+// With automatic rows, three steps per launch only where the batch's rows
+// make at least 8 per block at four blocks per CU.
+bool batch_three_steps_fill(int64_t rows_total, int cus);
+
 }  // namespace lbm

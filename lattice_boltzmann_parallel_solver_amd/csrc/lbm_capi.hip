@@ -981,7 +981,7 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
-        three_fill = rows_total / (4 * (int64_t)cus) >= 8;
+        three_fill = batch_three_steps_fill(rows_total, cus);
     }
     for (int j = 0; j < n; j++) {
         lbm_ctx *c = b->ctxs[j];
@@ -1058,20 +1058,13 @@ int batch_run_sweep(lbm_batch *b, int64_t nsteps) {
             lbm_ctx *c = b->ctxs[bk.members[m]];
             // short rows (blocks of a few waves) need many blocks in flight:
             // few rows per block; wide rows (512+ threads per block) take 32
-            LaunchCfg cfg = c->cfg;
-            cfg.sweep_rows = bk.threads >= 512 ? std::max(b->sweep_rows, 32) : b->sweep_rows;
             // three steps per launch read three halo rows on each side: four
             // times the rows per block (the 256-channel batch with its
             // three-step groups at 8, 16, 24, 32 and 48 rows: 132, 143, 147,
             // 149 and 148 GLUPS)
-            if (bk.k == 3) cfg.sweep_rows = std::max(cfg.sweep_rows, 4 * b->sweep_rows);
-            if (b->rows_auto) {
-                if (bk.k == 2 && bk.threads >= 512 && wide_rows > 0)
-                    cfg.sweep_rows = wide_rows;
-                else
-                    cfg.sweep_rows = (int)std::min<int64_t>(
-                        cfg.sweep_rows, std::max<int64_t>(1, rows_total / ((bk.threads >= 512 ? 2 : 4) * (int64_t)cus)));
-            }
+            LaunchCfg cfg = c->cfg;
+            cfg.sweep_rows =
+                batch_group_rows((int)bk.threads, bk.k, b->sweep_rows, b->rows_auto, rows_total, cus, wide_rows);
             uint32_t rows = (uint32_t)sweep_rows_for(c->g, cfg, 0);
             // the three-step kernel needs two rows in every block; the same
             // tables also serve its two-step launches

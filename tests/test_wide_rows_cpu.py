@@ -85,3 +85,56 @@ def test_random_batches_minimise_rounds_times_rows(tmp_path):
         costs = [_cost(lats, cus, q, halo) for q in range(1, max_rows + 1)]
         assert costs[r - 1] == min(costs)
         assert r == 1 + costs.index(min(costs))  # ties go to the fewest rows
+
+
+GROUP_DRIVER = r"""
+#include <cstdio>
+
+#include "lbm_batch_tables.h"
+
+int main() {
+    int threads, k, base, autorows, cus, wide;
+    long long total;
+    while (std::scanf("%d %d %d %d %lld %d %d", &threads, &k, &base, &autorows, &total, &cus, &wide) == 7)
+        std::printf("%d %d\n", lbm::batch_group_rows(threads, k, base, autorows != 0, total, cus, wide),
+                    lbm::batch_three_steps_fill(total, cus) ? 1 : 0);
+    return 0;
+}
+"""
+
+
+def test_group_rows_of_large_small_and_wide_batches(tmp_path):
+    """Rows per block asked of a batch group: the 256-channel batch (180,000
+    rows on 256 CUs) keeps 8 rows for narrow groups, 32 for wide ones and 32
+    for three-step groups; four channels of 2462 rows in all get 2 rows (four
+    blocks per CU) and too few rows for three steps per launch; wide groups
+    of a batch they dominate take the whole-round count; rows given by the
+    caller are only raised (32 for wide rows, 4x for three steps).  Groups of
+    512 threads or more count as wide (about two blocks per CU)."""
+    if shutil.which("g++") is None:
+        pytest.skip("g++ missing")
+    src = tmp_path / "group_rows.cpp"
+    src.write_text(GROUP_DRIVER)
+    exe = str(tmp_path / "group_rows")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", CSRC, "-o", exe, str(src),
+                    os.path.join(CSRC, "lbm_batch_tables.cpp")], check=True)
+    cases = [
+        # threads, k, base rows, automatic, rows in the batch, CUs, wide rows -> rows, three steps fill
+        (256, 2, 8, 1, 180000, 256, 0, 8, 1),
+        (576, 2, 8, 1, 180000, 256, 0, 32, 1),
+        (512, 3, 8, 1, 180000, 256, 0, 32, 1),
+        (256, 2, 8, 1, 2462, 256, 0, 2, 0),
+        (512, 3, 8, 1, 2462, 256, 0, 4, 0),    # 512 threads count as wide: two blocks per CU
+        (576, 2, 8, 1, 5819, 256, 0, 11, 0),
+        (1024, 2, 8, 1, 8192, 256, 32, 32, 1),
+        (1024, 2, 8, 1, 24576, 256, 97, 97, 1),
+        (256, 2, 5, 0, 100, 1, 0, 5, 1),
+        (1024, 2, 5, 0, 100, 1, 0, 32, 1),
+        (256, 3, 5, 0, 100, 1, 0, 20, 1),
+        (256, 2, 8, 1, 8191, 256, 0, 7, 0),   # just under 8 rows per block at four blocks per CU
+        (256, 2, 8, 1, 8192, 256, 0, 8, 1),
+    ]
+    text = "".join(f"{t} {k} {b} {a} {tot} {c} {w}\n" for t, k, b, a, tot, c, w, _, _ in cases)
+    out = subprocess.run([exe], input=text, capture_output=True, text=True, check=True).stdout.split()
+    got = [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(len(cases))]
+    assert got == [(r, f) for *_, r, f in cases]
